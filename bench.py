@@ -1,0 +1,203 @@
+"""Benchmark of the em3d hot path on one MI355X.
+
+One step = one EM matvec ``A u = u - B(chi u)`` at 128^3 cells (doubled FFT
+grid 256^3), complex64, anisotropic absorbing ellipsoid (radii 0.3/0.2/0.15,
+eps_r = diag(2.5+0.1i, 3.0+0.2i, 2.0), k0 = 8), synthetic Gaussian field,
+inputs resident in HBM.  Prints one JSON line with matvecs/s, the roofline of
+the dominant kernel (HIP events on the launch stream), solve times to true
+rtol 1e-8 and the CPU baseline (the NumPy oracle on a bounded sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu] [--no-solve]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-rtol 1e-8, learned-PC mixed vs plain FP64 BiCGStab @128³; matvec HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+PASS_NAMES = ["z_fwd_chi", "y_fwd", "x_fwd_khat_inv", "y_inv", "z_inv_epilogue"]
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def pass_bytes(N, es, n_chi):
+    """Algorithmic HBM bytes of each pass of one EM matvec (one RHS)."""
+    M = N ** 3
+    kq = 6 * (N + 1) ** 3
+    return [(9 + n_chi) * M * es, 18 * M * es, (24 * M + kq) * es, 18 * M * es, 12 * M * es]
+
+
+def make_problem(em, be, N, k0):
+    grid = em.Grid(N=(N, N, N), L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+    mask = em.geometry.center_mask(em.geometry.Ellipsoid(center=(0, 0, 0), radii=(0.3, 0.2, 0.15)), grid)
+    eps = np.diag([2.5 + 0.1j, 3.0 + 0.2j, 2.0])
+    chi = em.geometry.sample_contrast_tensor(grid, background_eps_r=1.0, feature_eps_r=eps,
+                                             geometry=em.geometry.Ellipsoid(center=(0, 0, 0),
+                                                                            radii=(0.3, 0.2, 0.15)))
+    wave = em.flat_wave_vec(grid, k=k0, orient=(0, 0, 1), amplitude=(1, 0, 0))
+    return em.Problem(grid=grid, eps_tensor=chi, wave=wave, k0=k0, volume=grid.dv * N ** 3), mask
+
+
+def cpu_baseline(N, reps, k0):
+    """Oracle (NumPy, complex64 as the reference's SINGLE path) matvec at the
+    same size; the spectrum is a random tensor of the reference's dense
+    (3,3,2N,2N,2N) shape, since the FFT and product cost do not depend on values."""
+    from oracle import vie
+    rng = np.random.default_rng(0)
+    shape = (3, 3) + (2 * N,) * 3
+    Kh = (rng.standard_normal(shape, dtype=np.float32) + 1j * rng.standard_normal(shape, dtype=np.float32))
+    Kh = Kh.astype(np.complex64)
+    chi = np.zeros((3, 3, N, N, N), np.complex64)
+    for a in range(3):
+        chi[a, a] = 1.0
+    u = (rng.standard_normal((3, N, N, N)) + 1j * rng.standard_normal((3, N, N, N))).astype(np.complex64)
+    vie.em_matvec(chi, u, Kh)  # warm-up
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        vie.em_matvec(chi, u, Kh)
+        t.append(time.perf_counter() - t0)
+    med = float(np.median(t))
+    return {"value": 1.0 / med, "unit": "matvecs/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} oracle EM matvecs at {N}^3 complex64 after 1 warm-up (median {med:.2f} s); "
+                      f"numpy.fft is single-threaded; host {platform.processor() or platform.machine()}, "
+                      f"os.cpu_count()={os.cpu_count()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--k0", type=float, default=8.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(the operator shards by right-hand side / material, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import em3d_amd as em
+    from em3d_amd import _lib
+
+    N, k0 = args.n, args.k0
+    be32 = em.Backend.hip(em.Precision.SINGLE)
+    log(f"[bench] building {N}^3 complex64 operator")
+    prob, mask = make_problem(em, be32, N, k0)
+    op = em.Operator(prob)
+    n_chi = {"isotropic": 1, "diagonal": 3, "full": 9}[op.contrast_kind]
+    rng = np.random.default_rng(42)
+    u = be32.array((rng.standard_normal((3, N, N, N)) + 1j * rng.standard_normal((3, N, N, N))).astype(np.complex64))
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        y = op.matvec(u)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        y = op.matvec(u)
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms_total = start.elapsed_time(end)
+    ms_per = ms_total / args.steps
+    log(f"[bench] matvec {ms_per:.3f} ms")
+
+    # per-launch durations with events recorded on the launch stream
+    from em3d_amd.operator import HipPlan
+    import ctypes as C
+    plan = HipPlan.get((N, N, N), 3, 1, em.Precision.SINGLE)
+    L = _lib.lib()
+    _lib.check(L.em3d_plan_set_timing(plan.handle, 1))
+    for _ in range(max(5, args.steps // 5)):
+        op.matvec(u)
+    buf = (C.c_double * 8)()
+    nl, nc = C.c_int(), C.c_longlong()
+    _lib.check(L.em3d_plan_timing(plan.handle, buf, 8, C.byref(nl), C.byref(nc)))
+    _lib.check(L.em3d_plan_set_timing(plan.handle, 0))
+    per_launch = [buf[i] for i in range(nl.value)]
+    es = 8
+    pb = pass_bytes(N, es, n_chi)
+    dom = int(np.argmax(per_launch))
+    achieved = pb[dom] / (per_launch[dom] * 1e-3) / 1e9
+    total_bytes = sum(pb)
+    log(f"[bench] per-launch ms {[round(x, 4) for x in per_launch]}")
+
+    solve = None
+    if not args.no_solve:
+        solve = {}
+        for name, prec in (("fp64_bicgstab", em.Precision.DOUBLE), ("fp32_bicgstab", em.Precision.SINGLE)):
+            be = em.Backend.hip(prec)
+            p2, _ = make_problem(em, be, N, k0)
+            o2 = em.Operator(p2)
+            cfg = em.SolverConfig(max_iter=500, rtol=1e-8 if prec is em.Precision.DOUBLE else 1e-5)
+            o2.matvec(p2.wave)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = em.BiCGStab(cfg).solve(o2, p2.wave)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if prec is em.Precision.DOUBLE:
+                r = o2.matvec(res.u) - p2.wave
+                true_res = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(p2.wave))
+            else:
+                true_res = float(res.residual_history[-1])
+            solve[name] = {"seconds": round(dt, 4), "iterations": res.iterations, "status": res.status,
+                           "rtol": cfg.rtol, "final_residual": true_res}
+            log(f"[bench] {name}: {dt:.3f} s, {res.iterations} it, {res.status}, res {true_res:.2e}")
+            del o2, p2, res
+            torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle, numpy)")
+        cpu = cpu_baseline(N, 2, k0)
+        log(f"[bench] CPU {cpu['value']:.4f} matvecs/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(1000.0 / ms_per, 3),
+        "unit": "matvecs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"EM matvec {N}^3 cells (FFT grid {2 * N}^3), diagonal absorbing ellipsoid "
+                               f"chi, k0={k0}, 1 RHS, complex64",
+                   "grid": N, "batch": 1, "chi": op.contrast_kind},
+        "matvec_algorithmic_bytes": total_bytes,
+        "matvec_hbm_gbs": round(total_bytes / (ms_per * 1e-3) / 1e9, 1),
+        "matvec_frac_hbm_peak": round(total_bytes / (ms_per * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "per_launch_ms": {PASS_NAMES[i]: round(x, 5) for i, x in enumerate(per_launch)},
+        "roofline": {"bound": "hbm", "kernel": PASS_NAMES[dom], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "solve": solve,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,68 @@
+"""ctypes binding of libem3d_hip.so (the C-ABI declared in include/em3d_hip.h).
+
+The product never falls back to another implementation: if the library is not
+built, or no GPU is visible, every compute entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libem3d_hip.so")
+
+# Every symbol the header declares, with (restype, argtypes).
+_VP, _I, _D, _LL, _SZ = C.c_void_p, C.c_int, C.c_double, C.c_longlong, C.c_size_t
+SIGNATURES = {
+    "em3d_version": (C.c_char_p, []),
+    "em3d_last_error": (C.c_char_p, []),
+    "em3d_plan_create": (_I, [_I, _I, _I, _I, _I, _I, C.POINTER(_VP)]),
+    "em3d_plan_destroy": (_I, [_VP]),
+    "em3d_plan_sizes": (_I, [_VP, C.POINTER(_SZ), C.POINTER(_SZ), C.POINTER(_SZ)]),
+    "em3d_kernel_build": (_I, [_VP, _D, _D, _D, _D, _VP, _VP, _VP]),
+    "em3d_matvec": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
+    "em3d_rmatvec": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
+    "em3d_plan_set_timing": (_I, [_VP, _I]),
+    "em3d_plan_timing": (_I, [_VP, C.POINTER(_D), _I, C.POINTER(_I), C.POINTER(_LL)]),
+    "em3d_reduce_work_bytes": (_SZ, []),
+    "em3d_dotc": (_I, [_I, _LL, _VP, _VP, _VP, _VP, _VP]),
+    "em3d_axpbypcz": (_I, [_I, _LL, _D, _D, _VP, _D, _D, _VP, _D, _D, _VP, _VP, _VP]),
+}
+
+_LIB = None
+
+
+class HipLibraryError(RuntimeError):
+    """The HIP library is missing or a call into it failed."""
+
+
+def lib():
+    """Load (once) and return the library, with every signature declared."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipLibraryError(
+                f"{LIB_PATH} is not built; run `python -m em3d_amd.build` (hipcc, gfx950)")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = handle
+    return _LIB
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().em3d_last_error().decode(errors="replace")
+        raise HipLibraryError(f"{what}: {msg}" if what else msg)
+
+
+def stream_handle(device=None) -> int:
+    """HIP stream of torch's current stream on the given device."""
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def ptr(t) -> int:
+    return t.data_ptr() if t is not None else None

@@ -1,0 +1,11 @@
+"""Scalar acoustic scattering on the HIP backend."""
+from __future__ import annotations
+
+from .kernel import prep_coeffs_acoustic, self_cell_coefficient
+from .materials import eta_ellipsoid, eta_homogeneous, eta_slab, eta_sphere
+from .operator import AcousticOperator
+from .problem import AcousticProblem, make_acoustic_problem, plane_wave_scalar
+
+__all__ = ["AcousticOperator", "AcousticProblem", "eta_ellipsoid", "eta_homogeneous", "eta_slab",
+           "eta_sphere", "make_acoustic_problem", "plane_wave_scalar", "prep_coeffs_acoustic",
+           "self_cell_coefficient"]

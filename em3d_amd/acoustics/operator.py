@@ -1,0 +1,32 @@
+"""Acoustic operator ``H u = u - B((eta - 1) u)`` on the HIP backend
+(src/em3d/acoustics/operator.py:25-73 of the reference)."""
+from __future__ import annotations
+
+from ..operator import CHI_ISO, _FieldApplier, require_hip
+from .kernel import prep_coeffs_acoustic
+from .problem import AcousticProblem
+
+
+class AcousticOperator(_FieldApplier):
+    def __init__(self, problem: AcousticProblem):
+        self.problem = problem
+        be = problem.grid.backend
+        require_hip(be)
+        self._init_applier(be, problem.grid.N, 1)
+        self._K_hat = prep_coeffs_acoustic(problem.grid, k0=problem.k0)
+        self._chi = problem.chi.contiguous()
+
+    @property
+    def backend(self):
+        return self._be
+
+    def matvec(self, u):
+        """``u - B(chi u)``."""
+        return self._apply(u, self._K_hat, CHI_ISO, self._chi, False)
+
+    def rmatvec(self, v):
+        """``v - conj(chi) B^H v``."""
+        return self._apply(v, self._K_hat, CHI_ISO, self._chi, True)
+
+    def to_dense(self):
+        raise RuntimeError("AcousticOperator.to_dense requires numpy backend")

@@ -1,0 +1,443 @@
+// Block-phase bodies of the generic (any 7-smooth length) FFT pass kernels.
+//
+// Each pass kernel is a sequence of phases separated by block barriers:
+//   load (global -> LDS, prologue) | Stockham stages (LDS ping-pong) |
+//   [spectral multiply] | [inverse stages] | store (LDS -> global, epilogue)
+// A phase is a function of (block id, thread id); the HIP kernels in
+// e3_kernels.hip run the phases with __syncthreads() between them.  Keeping
+// the phases as plain functions lets the CPU tests run the very same index
+// arithmetic thread by thread on the host.
+//
+// Layouts (C order, z fastest):
+//   field   u, y : (B, C, Nx, Ny, Nz)
+//   W1           : (B, C, Nx, Ny, Lz)      after the z pass
+//   W2           : (B, C, Nx, Ly, Lz)      after the y pass (x still unpadded)
+//   Khat octant  : (K, Nx+1, Ny+1, Nz+1)   K = 6 (xx,yy,zz,xy,xz,yz) or 1
+//   chi          : iso (Nx,Ny,Nz) | diag (3,Nx,Ny,Nz) | full (3,3,Nx,Ny,Nz)
+#pragma once
+
+#include "e3_common.hpp"
+
+namespace e3 {
+
+enum ChiKind { CHI_NONE = 0, CHI_ISO = 1, CHI_DIAG = 2, CHI_FULL = 3 };
+enum Prologue { PRO_COPY = 0, PRO_CHI = 1 };
+enum Epilogue { EPI_STORE = 0, EPI_SUB = 1, EPI_SUB_CHIH = 2 };
+
+// ---------------------------------------------------------------------------
+// Stockham stage over ncol pencils held in LDS (pencil p at base + p*ps).
+// Butterfly j of radix R with Ns = product of the radices already applied:
+//   v[m] = in[j + m L/R] * W_{Ns R}^{(j mod Ns) m};  v = DFT_R(v);
+//   out[(j / Ns) Ns R + (j mod Ns) + m Ns] = v[m]
+// ---------------------------------------------------------------------------
+template <typename T, int R, bool INV>
+E3_HD void stockham_stage(const cpx<T>* in, cpx<T>* out, int ncol, int ps, int L, int Ns,
+                          const cpx<T>* tw, int tid, int nt) {
+  const int LR = L / R;
+  const int nb = ncol * LR;
+  const int twstep = L / (Ns * R);
+  for (int b = tid; b < nb; b += nt) {
+    const int col = b / LR;
+    const int j = b - col * LR;
+    const cpx<T>* src = in + (size_t)col * ps;
+    cpx<T> v[R];
+    for (int m = 0; m < R; ++m) v[m] = src[j + m * LR];
+    const int k = j % Ns;
+    if (Ns > 1) {
+      for (int m = 1; m < R; ++m) {
+        cpx<T> w = tw[k * m * twstep];
+        if (INV) w = conj(w);
+        v[m] = v[m] * w;
+      }
+    }
+    dft_small<T, R, INV>(v, tw, L);
+    cpx<T>* dst = out + (size_t)col * ps + (j / Ns) * Ns * R + k;
+    for (int m = 0; m < R; ++m) dst[m * Ns] = v[m];
+  }
+}
+
+template <typename T, bool INV>
+E3_HD void stage_dispatch(int R, const cpx<T>* in, cpx<T>* out, int ncol, int ps, int L, int Ns,
+                          const cpx<T>* tw, int tid, int nt) {
+  switch (R) {
+    case 2: stockham_stage<T, 2, INV>(in, out, ncol, ps, L, Ns, tw, tid, nt); break;
+    case 3: stockham_stage<T, 3, INV>(in, out, ncol, ps, L, Ns, tw, tid, nt); break;
+    case 4: stockham_stage<T, 4, INV>(in, out, ncol, ps, L, Ns, tw, tid, nt); break;
+    case 5: stockham_stage<T, 5, INV>(in, out, ncol, ps, L, Ns, tw, tid, nt); break;
+    case 7: stockham_stage<T, 7, INV>(in, out, ncol, ps, L, Ns, tw, tid, nt); break;
+    case 8: stockham_stage<T, 8, INV>(in, out, ncol, ps, L, Ns, tw, tid, nt); break;
+    default: break;
+  }
+}
+
+// Stage s of a plan: reads buffer (s & 1), writes buffer (s & 1) ^ 1.
+template <typename T>
+E3_HD void fft_stage(const AxisPlan& pl, int s, bool inv, cpx<T>* b0, cpx<T>* b1, int ncol, int ps,
+                     const cpx<T>* tw, int tid, int nt) {
+  int Ns = 1;
+  for (int i = 0; i < s; ++i) Ns *= pl.rad[i];
+  const cpx<T>* in = (s & 1) ? b1 : b0;
+  cpx<T>* out = (s & 1) ? b0 : b1;
+  if (inv) stage_dispatch<T, true>(pl.rad[s], in, out, ncol, ps, pl.L, Ns, tw, tid, nt);
+  else stage_dispatch<T, false>(pl.rad[s], in, out, ncol, ps, pl.L, Ns, tw, tid, nt);
+}
+
+// Buffer that holds the result after all stages of a plan.
+template <typename T>
+E3_HD cpx<T>* fft_result(const AxisPlan& pl, cpx<T>* b0, cpx<T>* b1) {
+  return (pl.nst & 1) ? b1 : b0;
+}
+
+// ---------------------------------------------------------------------------
+// contrast helpers: w = chi u, w = chi^H t at one cell
+// ---------------------------------------------------------------------------
+template <typename T>
+E3_HD void chi_apply(const cpx<T>* chi, int kind, long long ncell, long long cell, int C,
+                     const cpx<T>* u, cpx<T>* w, bool herm) {
+  if (kind == CHI_NONE) {
+    for (int c = 0; c < C; ++c) w[c] = u[c];
+    return;
+  }
+  if (kind == CHI_ISO) {
+    cpx<T> x = chi[cell];
+    if (herm) x = conj(x);
+    for (int c = 0; c < C; ++c) w[c] = x * u[c];
+    return;
+  }
+  if (kind == CHI_DIAG) {
+    for (int c = 0; c < C; ++c) {
+      cpx<T> x = chi[c * ncell + cell];
+      if (herm) x = conj(x);
+      w[c] = x * u[c];
+    }
+    return;
+  }
+  // full 3x3: w_a = sum_b chi_ab u_b, or (chi^H t)_a = sum_b conj(chi_ba) t_b
+  for (int a = 0; a < 3; ++a) {
+    cpx<T> acc = czero<T>();
+    for (int b = 0; b < 3; ++b) {
+      if (herm) acc = acc + cmulc(chi[(b * 3 + a) * ncell + cell], u[b]);
+      else acc = acc + chi[(a * 3 + b) * ncell + cell] * u[b];
+    }
+    w[a] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Z pass: transform along the contiguous axis, C components per pencil.
+// ---------------------------------------------------------------------------
+struct ZArgs {
+  const void* in;
+  void* out;
+  int C;
+  long long P;                 // pencils per component
+  long long in_ps, in_cs, in_bs;
+  long long out_ps, out_cs, out_bs;
+  int n_in, n_out;             // live input length, kept output length
+  int T;                       // pencils per block
+  int ps;                      // LDS pencil stride
+  AxisPlan plan;
+  const void* tw;
+  int inv;
+  int pro;                     // Prologue
+  int epi;                     // Epilogue
+  const void* chi;
+  int chi_kind;
+  long long ncell;
+  const void* u;               // epilogue source, same layout as out
+  int batch;
+};
+
+template <typename T>
+struct ZPass {
+  const ZArgs& a;
+  long long p0;
+  int b;
+  cpx<T>* b0;
+  cpx<T>* b1;
+  E3_HD ZPass(const ZArgs& args, long long blk, int bidx, cpx<T>* lds)
+      : a(args), p0(blk * args.T), b(bidx), b0(lds), b1(lds + (size_t)args.C * args.T * args.ps) {}
+
+  E3_HD void load(int tid, int nt) const {
+    const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs;
+    const int L = a.plan.L;
+    for (int e = tid; e < a.T * L; e += nt) {
+      const int t = e / L, z = e - t * L;
+      const long long p = p0 + t;
+      cpx<T> w[6];
+      if (p < a.P && z < a.n_in) {
+        cpx<T> v[6];
+        for (int c = 0; c < a.C; ++c) v[c] = in[c * a.in_cs + p * a.in_ps + z];
+        if (a.pro == PRO_CHI)
+          chi_apply<T>((const cpx<T>*)a.chi, a.chi_kind, a.ncell, p * a.n_in + z, a.C, v, w, false);
+        else
+          for (int c = 0; c < a.C; ++c) w[c] = v[c];
+      } else {
+        for (int c = 0; c < a.C; ++c) w[c] = czero<T>();
+      }
+      for (int c = 0; c < a.C; ++c) b0[(size_t)(c * a.T + t) * a.ps + z] = w[c];
+    }
+  }
+  E3_HD void stage(int s, int tid, int nt) const {
+    fft_stage<T>(a.plan, s, a.inv != 0, b0, b1, a.C * a.T, a.ps, (const cpx<T>*)a.tw, tid, nt);
+  }
+  E3_HD void store(int tid, int nt) const {
+    const cpx<T>* res = fft_result<T>(a.plan, b0, b1);
+    cpx<T>* out = (cpx<T>*)a.out + (size_t)b * a.out_bs;
+    const cpx<T>* src = (const cpx<T>*)a.u + (size_t)b * a.out_bs;
+    for (int e = tid; e < a.T * a.n_out; e += nt) {
+      const int t = e / a.n_out, z = e - t * a.n_out;
+      const long long p = p0 + t;
+      if (p >= a.P) continue;
+      cpx<T> v[6];
+      for (int c = 0; c < a.C; ++c) v[c] = res[(size_t)(c * a.T + t) * a.ps + z];
+      if (a.epi == EPI_SUB_CHIH) {
+        cpx<T> w[6];
+        chi_apply<T>((const cpx<T>*)a.chi, a.chi_kind, a.ncell, p * a.n_out + z, a.C, v, w, true);
+        for (int c = 0; c < a.C; ++c) v[c] = w[c];
+      }
+      for (int c = 0; c < a.C; ++c) {
+        const size_t o = c * a.out_cs + p * a.out_ps + z;
+        out[o] = (a.epi == EPI_STORE) ? v[c] : src[o] - v[c];
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// S pass: transform along a strided axis.  Data viewed as (Q, rows, W) with
+// row stride W; a block takes T consecutive columns of one plane.
+// ---------------------------------------------------------------------------
+struct SArgs {
+  const void* in;
+  void* out;
+  long long Q, W;
+  long long in_qs, out_qs;
+  int n_in, n_out;
+  int T, ps;
+  AxisPlan plan;
+  const void* tw;
+  int inv;
+};
+
+template <typename T>
+struct SPass {
+  const SArgs& a;
+  long long q, w0;
+  cpx<T>* b0;
+  cpx<T>* b1;
+  E3_HD SPass(const SArgs& args, long long colblk, long long plane, cpx<T>* lds)
+      : a(args), q(plane), w0(colblk * args.T), b0(lds), b1(lds + (size_t)args.T * args.ps) {}
+
+  E3_HD void load(int tid, int nt) const {
+    const cpx<T>* in = (const cpx<T>*)a.in + (size_t)q * a.in_qs;
+    const int L = a.plan.L;
+    for (int e = tid; e < a.T * L; e += nt) {
+      const int row = e / a.T, col = e - row * a.T;
+      const long long w = w0 + col;
+      cpx<T> v = czero<T>();
+      if (row < a.n_in && w < a.W) v = in[(size_t)row * a.W + w];
+      b0[(size_t)col * a.ps + row] = v;
+    }
+  }
+  E3_HD void stage(int s, int tid, int nt) const {
+    fft_stage<T>(a.plan, s, a.inv != 0, b0, b1, a.T, a.ps, (const cpx<T>*)a.tw, tid, nt);
+  }
+  E3_HD void store(int tid, int nt) const {
+    const cpx<T>* res = fft_result<T>(a.plan, b0, b1);
+    cpx<T>* out = (cpx<T>*)a.out + (size_t)q * a.out_qs;
+    for (int e = tid; e < a.T * a.n_out; e += nt) {
+      const int row = e / a.T, col = e - row * a.T;
+      const long long w = w0 + col;
+      if (w < a.W) out[(size_t)row * a.W + w] = res[(size_t)col * a.ps + row];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// X pass: forward transform along x (Nx live rows), multiply by the octant
+// spectrum (3x3 symmetric or scalar), inverse transform, keep Nx rows.  In
+// place on W2 viewed per (b, c) as (Nx, W = Ly*Lz).
+// ---------------------------------------------------------------------------
+struct XArgs {
+  void* data;
+  int C;
+  long long W, cs, bs;
+  int N[3], L[3];
+  int T, ps;
+  AxisPlan plan;
+  const void* tw;
+  const void* khat;
+  int adj;
+};
+
+template <typename T>
+E3_HD void spectral_multiply(const cpx<T>* K, int C, int N[3], int L[3], int kx, int ky, int kz,
+                             bool adj, cpx<T>* f) {
+  int fx, fy, fz;
+  const int ix = fold_index(kx, N[0], L[0], &fx);
+  const int iy = fold_index(ky, N[1], L[1], &fy);
+  const int iz = fold_index(kz, N[2], L[2], &fz);
+  const size_t oct = (size_t)(N[0] + 1) * (N[1] + 1) * (N[2] + 1);
+  const size_t at = ((size_t)ix * (N[1] + 1) + iy) * (N[2] + 1) + iz;
+  if (C == 1) {
+    cpx<T> k = K[at];
+    if (adj) k = conj(k);
+    f[0] = k * f[0];
+    return;
+  }
+  cpx<T> k[6];
+  for (int i = 0; i < 6; ++i) {
+    k[i] = K[i * oct + at];
+    if (adj) k[i] = conj(k[i]);
+  }
+  const T sx = fx ? T(-1) : T(1), sy = fy ? T(-1) : T(1), sz = fz ? T(-1) : T(1);
+  const cpx<T> kxy = (sx * sy) * k[3], kxz = (sx * sz) * k[4], kyz = (sy * sz) * k[5];
+  const cpx<T> f0 = f[0], f1 = f[1], f2 = f[2];
+  f[0] = k[0] * f0 + kxy * f1 + kxz * f2;
+  f[1] = kxy * f0 + k[1] * f1 + kyz * f2;
+  f[2] = kxz * f0 + kyz * f1 + k[2] * f2;
+}
+
+template <typename T>
+struct XPass {
+  const XArgs& a;
+  long long w0;
+  int b;
+  cpx<T>* b0;
+  cpx<T>* b1;
+  E3_HD XPass(const XArgs& args, long long colblk, int bidx, cpx<T>* lds)
+      : a(args), w0(colblk * args.T), b(bidx), b0(lds), b1(lds + (size_t)args.C * args.T * args.ps) {}
+
+  E3_HD void load(int tid, int nt) const {
+    const cpx<T>* d = (const cpx<T>*)a.data + (size_t)b * a.bs;
+    const int L = a.plan.L, TL = a.T * L;
+    for (int e = tid; e < a.C * TL; e += nt) {
+      const int c = e / TL, r = e - c * TL;
+      const int row = r / a.T, col = r - row * a.T;
+      const long long w = w0 + col;
+      cpx<T> v = czero<T>();
+      if (row < a.N[0] && w < a.W) v = d[(size_t)c * a.cs + (size_t)row * a.W + w];
+      b0[(size_t)(c * a.T + col) * a.ps + row] = v;
+    }
+  }
+  E3_HD void stage_fwd(int s, int tid, int nt) const {
+    fft_stage<T>(a.plan, s, false, b0, b1, a.C * a.T, a.ps, (const cpx<T>*)a.tw, tid, nt);
+  }
+  // After the forward stages the spectrum sits in fft_result(b0,b1); the
+  // multiply runs in place there and the inverse stages continue from it.
+  E3_HD cpx<T>* spec() const { return fft_result<T>(a.plan, b0, b1); }
+  E3_HD void multiply(int tid, int nt) const {
+    cpx<T>* sp = spec();
+    const int L = a.plan.L;
+    for (int e = tid; e < a.T * L; e += nt) {
+      const int col = e / L, kx = e - col * L;
+      const long long w = w0 + col;
+      if (w >= a.W) continue;
+      const int ky = (int)(w / a.L[2]), kz = (int)(w - (long long)ky * a.L[2]);
+      cpx<T> f[3];
+      for (int c = 0; c < a.C; ++c) f[c] = sp[(size_t)(c * a.T + col) * a.ps + kx];
+      int N[3] = {a.N[0], a.N[1], a.N[2]}, Ld[3] = {a.L[0], a.L[1], a.L[2]};
+      spectral_multiply<T>((const cpx<T>*)a.khat, a.C, N, Ld, kx, ky, kz, a.adj != 0, f);
+      for (int c = 0; c < a.C; ++c) sp[(size_t)(c * a.T + col) * a.ps + kx] = f[c];
+    }
+  }
+  // inverse stage s reads/writes starting from the spectrum buffer
+  E3_HD void stage_inv(int s, int tid, int nt) const {
+    cpx<T>* first = spec();
+    cpx<T>* other = (first == b0) ? b1 : b0;
+    fft_stage<T>(a.plan, s, true, first, other, a.C * a.T, a.ps, (const cpx<T>*)a.tw, tid, nt);
+  }
+  E3_HD void store(int tid, int nt) const {
+    cpx<T>* first = spec();
+    cpx<T>* other = (first == b0) ? b1 : b0;
+    const cpx<T>* res = fft_result<T>(a.plan, first, other);
+    cpx<T>* d = (cpx<T>*)a.data + (size_t)b * a.bs;
+    const int TN = a.T * a.N[0];
+    for (int e = tid; e < a.C * TN; e += nt) {
+      const int c = e / TN, r = e - c * TN;
+      const int row = r / a.T, col = r - row * a.T;
+      const long long w = w0 + col;
+      if (w < a.W) d[(size_t)c * a.cs + (size_t)row * a.W + w] = res[(size_t)(c * a.T + col) * a.ps + row];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Kernel build (FP64): spatial kernel on the doubled grid with the index-N
+// planes set to zero, then (after a full 3-D FFT) the octant extraction.
+// ---------------------------------------------------------------------------
+struct KBArgs {
+  void* full;          // complex<double> (K, Lx, Ly, Lz)
+  void* oct;           // complex<T> (K, Nx+1, Ny+1, Nz+1)
+  int kind;            // 0 EM (K = 6), 1 acoustic (K = 1)
+  double k0, h[3], dv;
+  double self_re, self_im;   // acoustic self-cell value
+  int N[3], L[3];
+  double scale;        // 1 / (Lx Ly Lz)
+};
+
+E3_HD double grid_offset(int i, int N, double h) { return (i < N) ? i * h : (i - 2 * N) * h; }
+
+E3_HD void kernel_point(const KBArgs& a, long long idx) {
+  cpx<double>* out = (cpx<double>*)a.full;
+  const long long D = (long long)a.L[0] * a.L[1] * a.L[2];
+  const int iz = (int)(idx % a.L[2]);
+  const int iy = (int)((idx / a.L[2]) % a.L[1]);
+  const int ix = (int)(idx / ((long long)a.L[2] * a.L[1]));
+  const int ncomp = a.kind == 0 ? 6 : 1;
+  const bool nyq = (ix == a.N[0]) || (iy == a.N[1]) || (iz == a.N[2]);
+  if (nyq) {
+    for (int c = 0; c < ncomp; ++c) out[c * D + idx] = mk(0.0, 0.0);
+    return;
+  }
+  const double s[3] = {grid_offset(ix, a.N[0], a.h[0]), grid_offset(iy, a.N[1], a.h[1]),
+                       grid_offset(iz, a.N[2], a.h[2])};
+  const double R = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+  const double k = a.k0;
+  if (R < 1e-15) {
+    if (a.kind == 0) {
+      for (int c = 0; c < 3; ++c) out[c * D + idx] = mk(-1.0 / 3.0, 0.0);
+      for (int c = 3; c < 6; ++c) out[c * D + idx] = mk(0.0, 0.0);
+    } else {
+      out[idx] = mk(a.self_re, a.self_im);
+    }
+    return;
+  }
+  const double inv = 1.0 / R;
+  const double g = 1.0 / (4.0 * 3.14159265358979323846 * R);
+  const cpx<double> e = mk(cos(k * R) * g, sin(k * R) * g);   // exp(ikR)/(4 pi R)
+  if (a.kind == 1) {
+    out[idx] = (k * k * a.dv) * e;
+    return;
+  }
+  const cpx<double> wg = a.dv * e;
+  const cpx<double> c1 = mk(3.0 * inv * inv - k * k, -3.0 * k * inv);
+  const cpx<double> c2 = mk(k * k - inv * inv, k * inv);
+  const cpx<double> w1 = wg * c1, w2 = wg * c2;
+  const double al[3] = {s[0] * inv, s[1] * inv, s[2] * inv};
+  out[0 * D + idx] = (al[0] * al[0]) * w1 + w2;
+  out[1 * D + idx] = (al[1] * al[1]) * w1 + w2;
+  out[2 * D + idx] = (al[2] * al[2]) * w1 + w2;
+  out[3 * D + idx] = (al[0] * al[1]) * w1;
+  out[4 * D + idx] = (al[0] * al[2]) * w1;
+  out[5 * D + idx] = (al[1] * al[2]) * w1;
+}
+
+template <typename T>
+E3_HD void octant_point(const KBArgs& a, long long idx) {
+  const cpx<double>* full = (const cpx<double>*)a.full;
+  cpx<T>* oct = (cpx<T>*)a.oct;
+  const long long nz1 = a.N[2] + 1, ny1 = a.N[1] + 1, nx1 = a.N[0] + 1;
+  const long long per = nx1 * ny1 * nz1;
+  const int c = (int)(idx / per);
+  const long long r = idx - c * per;
+  const int iz = (int)(r % nz1);
+  const int iy = (int)((r / nz1) % ny1);
+  const int ix = (int)(r / (nz1 * ny1));
+  const long long D = (long long)a.L[0] * a.L[1] * a.L[2];
+  const cpx<double> v = full[c * D + ((long long)ix * a.L[1] + iy) * a.L[2] + iz];
+  oct[idx] = mk((T)(v.re * a.scale), (T)(v.im * a.scale));
+}
+
+}  // namespace e3

@@ -1,0 +1,334 @@
+// libem3d_hip: HIP kernels of the FFT volume-integral operator and the C-ABI.
+//
+// Entry points are declared (with the reference interface each one replaces)
+// in include/em3d_hip.h.  Rules of the ABI: every function returns int status
+// (0 = OK, else the message is in em3d_last_error()); no C++ exception crosses
+// it; all data pointers are device pointers owned by the caller; the stream is
+// the caller's.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/em3d_hip.h"
+#include "e3_error.hpp"
+#include "e3_pipeline.hpp"
+
+using namespace e3;
+
+namespace {
+
+constexpr int kMaxTimedLaunches = 8;
+
+int fail(const std::string& msg) { return set_error(msg); }
+
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  return fail(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+extern "C" __shared__ double2 e3_smem[];
+
+// ---------------------------------------------------------------------------
+// pass kernels: phases of e3_blocks.hpp separated by barriers
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_zpass(ZArgs a, long long nblk) {
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(e3_smem);
+  const long long bid = blockIdx.x;
+  const int b = (int)(bid / nblk);
+  ZPass<T> z(a, bid - (long long)b * nblk, b, lds);
+  z.load(threadIdx.x, blockDim.x);
+  __syncthreads();
+  for (int s = 0; s < a.plan.nst; ++s) {
+    z.stage(s, threadIdx.x, blockDim.x);
+    __syncthreads();
+  }
+  z.store(threadIdx.x, blockDim.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_spass(SArgs a, long long ncb) {
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(e3_smem);
+  const long long bid = blockIdx.x;
+  const long long q = bid / ncb;
+  SPass<T> s(a, bid - q * ncb, q, lds);
+  s.load(threadIdx.x, blockDim.x);
+  __syncthreads();
+  for (int i = 0; i < a.plan.nst; ++i) {
+    s.stage(i, threadIdx.x, blockDim.x);
+    __syncthreads();
+  }
+  s.store(threadIdx.x, blockDim.x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_xpass(XArgs a, long long ncb) {
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(e3_smem);
+  const long long bid = blockIdx.x;
+  const int b = (int)(bid / ncb);
+  XPass<T> x(a, bid - (long long)b * ncb, b, lds);
+  x.load(threadIdx.x, blockDim.x);
+  __syncthreads();
+  for (int i = 0; i < a.plan.nst; ++i) {
+    x.stage_fwd(i, threadIdx.x, blockDim.x);
+    __syncthreads();
+  }
+  x.multiply(threadIdx.x, blockDim.x);
+  __syncthreads();
+  for (int i = 0; i < a.plan.nst; ++i) {
+    x.stage_inv(i, threadIdx.x, blockDim.x);
+    __syncthreads();
+  }
+  x.store(threadIdx.x, blockDim.x);
+}
+
+__global__ __launch_bounds__(256) void k_kspatial(KBArgs a, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) kernel_point(a, i);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_koct(KBArgs a, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) octant_point<T>(a, i);
+}
+
+template <typename K>
+void allow_big_lds(K kernel) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
+}
+
+struct HipLauncher {
+  hipStream_t st;
+  hipError_t err = hipSuccess;
+  hipEvent_t* ev = nullptr;   // when set, ev[i] is recorded before launch i
+  int launches = 0;
+  void note(hipError_t e) {
+    if (err == hipSuccess && e != hipSuccess) err = e;
+  }
+  void mark() {
+    if (ev && launches < kMaxTimedLaunches) note(hipEventRecord(ev[launches], st));
+    ++launches;
+  }
+  void z(const ZArgs& a, long long nblk, int batch, int prec, size_t lds) {
+    mark();
+    const dim3 g((unsigned)(nblk * batch));
+    if (prec == PREC_DOUBLE) hipLaunchKernelGGL(k_zpass<double>, g, dim3(kThreads), lds, st, a, nblk);
+    else hipLaunchKernelGGL(k_zpass<float>, g, dim3(kThreads), lds, st, a, nblk);
+    note(hipGetLastError());
+  }
+  void s(const SArgs& a, long long ncb, long long Q, int prec, size_t lds) {
+    mark();
+    const dim3 g((unsigned)(ncb * Q));
+    if (prec == PREC_DOUBLE) hipLaunchKernelGGL(k_spass<double>, g, dim3(kThreads), lds, st, a, ncb);
+    else hipLaunchKernelGGL(k_spass<float>, g, dim3(kThreads), lds, st, a, ncb);
+    note(hipGetLastError());
+  }
+  void x(const XArgs& a, long long ncb, int batch, int prec, size_t lds) {
+    mark();
+    const dim3 g((unsigned)(ncb * batch));
+    if (prec == PREC_DOUBLE) hipLaunchKernelGGL(k_xpass<double>, g, dim3(kThreads), lds, st, a, ncb);
+    else hipLaunchKernelGGL(k_xpass<float>, g, dim3(kThreads), lds, st, a, ncb);
+    note(hipGetLastError());
+  }
+  void kspatial(const KBArgs& a, long long n) {
+    mark();
+    hipLaunchKernelGGL(k_kspatial, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n);
+    note(hipGetLastError());
+  }
+  void koct(const KBArgs& a, long long n, int prec) {
+    mark();
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (prec == PREC_DOUBLE) hipLaunchKernelGGL(k_koct<double>, g, dim3(256), 0, st, a, n);
+    else hipLaunchKernelGGL(k_koct<float>, g, dim3(256), 0, st, a, n);
+    note(hipGetLastError());
+  }
+};
+
+void set_lds_limits_once() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  allow_big_lds(k_zpass<float>);
+  allow_big_lds(k_zpass<double>);
+  allow_big_lds(k_spass<float>);
+  allow_big_lds(k_spass<double>);
+  allow_big_lds(k_xpass<float>);
+  allow_big_lds(k_xpass<double>);
+}
+
+}  // namespace
+
+struct em3d_plan_s {
+  Plan p;
+  void* tw_dev = nullptr;
+  int device = 0;
+  // per-pass timing (em3d_plan_set_timing): events around the launches of
+  // each apply call and the accumulated elapsed milliseconds per launch slot
+  bool timing = false;
+  hipEvent_t ev[kMaxTimedLaunches + 1] = {};
+  double ms[kMaxTimedLaunches] = {};
+  long long calls = 0;
+  int nlaunch = 0;
+};
+
+extern "C" {
+
+const char* em3d_version(void) { return "em3d_hip 0.1.0 (gfx950)"; }
+
+const char* em3d_last_error(void) { return last_error_store().c_str(); }
+
+int em3d_plan_create(int nx, int ny, int nz, int ncomp, int batch, int precision, em3d_plan_t* out) {
+  if (!out) return fail("em3d_plan_create: null output handle");
+  *out = nullptr;
+  em3d_plan_s* h = new (std::nothrow) em3d_plan_s();
+  if (!h) return fail("em3d_plan_create: out of host memory");
+  const std::string why = plan_init(&h->p, nx, ny, nz, ncomp, batch, precision);
+  if (!why.empty()) {
+    delete h;
+    return fail("em3d_plan_create: " + why);
+  }
+  // twiddle tables exp(-2 pi i t / L) for the three axes, float then double
+  size_t nf = 0;
+  for (int a = 0; a < 3; ++a) nf += (size_t)h->p.L[a];
+  std::vector<float> tf(2 * nf);
+  std::vector<double> td(2 * nf);
+  size_t off = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int L = h->p.L[a];
+    for (int t = 0; t < L; ++t) {
+      const double ang = -2.0 * M_PI * (double)t / (double)L;
+      td[2 * (off + t)] = std::cos(ang);
+      td[2 * (off + t) + 1] = std::sin(ang);
+      tf[2 * (off + t)] = (float)std::cos(ang);
+      tf[2 * (off + t) + 1] = (float)std::sin(ang);
+    }
+    off += (size_t)L;
+  }
+  const size_t bytes_f = tf.size() * sizeof(float), bytes_d = td.size() * sizeof(double);
+  if (hip_check(hipGetDevice(&h->device), "hipGetDevice") ||
+      hip_check(hipMalloc(&h->tw_dev, bytes_d + bytes_f), "hipMalloc(twiddles)")) {
+    delete h;
+    return 1;
+  }
+  char* base = (char*)h->tw_dev;
+  if (hip_check(hipMemcpy(base, td.data(), bytes_d, hipMemcpyHostToDevice), "hipMemcpy(twiddles)") ||
+      hip_check(hipMemcpy(base + bytes_d, tf.data(), bytes_f, hipMemcpyHostToDevice), "hipMemcpy(twiddles)")) {
+    (void)hipFree(h->tw_dev);
+    delete h;
+    return 1;
+  }
+  off = 0;
+  for (int a = 0; a < 3; ++a) {
+    h->p.tw_d[a] = base + off * 16;
+    h->p.tw_f[a] = base + bytes_d + off * 8;
+    off += (size_t)h->p.L[a];
+  }
+  set_lds_limits_once();
+  *out = h;
+  return 0;
+}
+
+int em3d_plan_set_timing(em3d_plan_t plan, int enable) {
+  if (!plan) return fail("em3d_plan_set_timing: null plan");
+  if (enable && !plan->timing) {
+    for (int i = 0; i <= kMaxTimedLaunches; ++i)
+      if (!plan->ev[i] && hip_check(hipEventCreate(&plan->ev[i]), "hipEventCreate")) return 1;
+  }
+  plan->timing = enable != 0;
+  for (int i = 0; i < kMaxTimedLaunches; ++i) plan->ms[i] = 0.0;
+  plan->calls = 0;
+  plan->nlaunch = 0;
+  return 0;
+}
+
+int em3d_plan_timing(em3d_plan_t plan, double* ms_per_launch, int max_launches, int* nlaunch, long long* calls) {
+  if (!plan || !ms_per_launch || !nlaunch || !calls) return fail("em3d_plan_timing: null argument");
+  *nlaunch = plan->nlaunch;
+  *calls = plan->calls;
+  for (int i = 0; i < plan->nlaunch && i < max_launches; ++i)
+    ms_per_launch[i] = plan->calls ? plan->ms[i] / (double)plan->calls : 0.0;
+  return 0;
+}
+
+int em3d_plan_destroy(em3d_plan_t plan) {
+  if (!plan) return 0;
+  int rc = 0;
+  for (int i = 0; i <= kMaxTimedLaunches; ++i)
+    if (plan->ev[i]) (void)hipEventDestroy(plan->ev[i]);
+  if (plan->tw_dev) rc = hip_check(hipFree(plan->tw_dev), "hipFree(twiddles)");
+  delete plan;
+  return rc;
+}
+
+int em3d_plan_sizes(em3d_plan_t plan, size_t* khat_bytes, size_t* work_bytes, size_t* build_bytes) {
+  if (!plan) return fail("em3d_plan_sizes: null plan");
+  const size_t es = elem_bytes(plan->p.prec);
+  if (khat_bytes) *khat_bytes = (size_t)plan->p.khat_elems * es;
+  if (work_bytes) *work_bytes = (size_t)(plan->p.w1_elems + plan->p.w2_elems) * es;
+  if (build_bytes) *build_bytes = (size_t)plan->p.build_elems * 16;
+  return 0;
+}
+
+int em3d_kernel_build(em3d_plan_t plan, double k0, double hx, double hy, double hz, void* khat,
+                      void* build_work, hipStream_t stream) {
+  if (!plan || !khat || !build_work) return fail("em3d_kernel_build: null argument");
+  if (!(k0 > 0.0)) return fail("em3d_kernel_build: k0 must be positive");
+  if (!(hx > 0.0 && hy > 0.0 && hz > 0.0)) return fail("em3d_kernel_build: cell sizes must be positive");
+  const double h[3] = {hx, hy, hz};
+  double sre = 0.0, sim = 0.0;
+  if (plan->p.C == 1) {
+    // equivalent-sphere self cell exp(ika)(1 - ika) - 1, a = (3 dv / 4 pi)^(1/3)
+    const double dv = hx * hy * hz;
+    const double a = std::cbrt(3.0 * dv / (4.0 * M_PI));
+    const double ka = k0 * a;
+    const double cr = std::cos(ka), ci = std::sin(ka);
+    sre = cr + ci * ka - 1.0;
+    sim = ci - cr * ka;
+  }
+  HipLauncher go{stream};
+  run_kernel_build(go, plan->p, k0, h, sre, sim, khat, build_work);
+  if (go.err != hipSuccess) return hip_check(go.err, "em3d_kernel_build launch");
+  return 0;
+}
+
+static int apply(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
+                 void* work, hipStream_t stream, bool adjoint, const char* name) {
+  if (!plan || !khat || !u || !y || !work) return fail(std::string(name) + ": null argument");
+  if (chi_kind < CHI_NONE || chi_kind > CHI_FULL) return fail(std::string(name) + ": bad chi_kind");
+  if (chi_kind != CHI_NONE && !chi) return fail(std::string(name) + ": null chi");
+  if (plan->p.C == 1 && chi_kind > CHI_ISO) return fail(std::string(name) + ": acoustic contrast must be scalar");
+  if (u == y) return fail(std::string(name) + ": output must not alias the input");
+  HipLauncher go{stream};
+  if (plan->timing) go.ev = plan->ev;
+  run_apply(go, plan->p, khat, chi, chi_kind, u, y, work, adjoint);
+  if (plan->timing && go.launches <= kMaxTimedLaunches) {
+    go.note(hipEventRecord(plan->ev[go.launches], stream));
+    go.note(hipEventSynchronize(plan->ev[go.launches]));
+    for (int i = 0; i < go.launches; ++i) {
+      float t = 0.f;
+      go.note(hipEventElapsedTime(&t, plan->ev[i], plan->ev[i + 1]));
+      plan->ms[i] += t;
+    }
+    plan->nlaunch = go.launches;
+    plan->calls += 1;
+  }
+  if (go.err != hipSuccess) return hip_check(go.err, name);
+  return 0;
+}
+
+int em3d_matvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
+                void* work, hipStream_t stream) {
+  return apply(plan, khat, chi, chi_kind, u, y, work, stream, false, "em3d_matvec");
+}
+
+int em3d_rmatvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* v, void* y,
+                 void* work, hipStream_t stream) {
+  return apply(plan, khat, chi, chi_kind, v, y, work, stream, true, "em3d_rmatvec");
+}
+
+}  // extern "C"

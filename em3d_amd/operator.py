@@ -1,0 +1,331 @@
+"""FFT volume-integral operator on MI355X (src/em3d/operator.py of the reference).
+
+``PreparedEMKernel`` and ``Operator`` keep the reference's constructor
+signatures, validation rules and error types; the arithmetic runs in
+libem3d_hip.so:
+
+* the kernel spectrum is built on the device in FP64 (spatial dyadic blocks,
+  3-D FFT) and held as the octant of its 6 unique symmetric components
+  (the spectrum of an even/odd block is even/odd per axis once the index-N
+  planes of the doubled grid, which the cropped convolution never reads, are
+  zeroed);
+* ``matvec``/``rmatvec`` run the five pruned FFT passes with the contrast
+  product, the 3x3 spectral product and the ``u - .`` epilogue fused in.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from math import prod
+from typing import Literal
+
+import numpy as np
+
+from . import _lib
+from .dtypes import Precision
+from .grid import Grid
+from .layout import flatten_field, unflatten_field
+from .problem import Problem
+
+AdjointStorage = Literal["explicit", "derived", "none"]
+KernelBuildStrategy = Literal["standard", "streamed"]
+
+CHI_NONE, CHI_ISO, CHI_DIAG, CHI_FULL = 0, 1, 2, 3
+_OCT_COMPONENTS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+
+
+def require_hip(backend) -> None:
+    if getattr(backend, "device", None) != "hip":
+        raise TypeError("em3d_amd computes on Backend.hip() only (got device "
+                        f"{getattr(backend, 'device', None)!r}); use the em3d package for NumPy/CuPy")
+
+
+class HipPlan:
+    """Owner of one ``em3d_plan_t`` (radix plans and twiddles of a grid shape)."""
+
+    _cache: dict = {}
+
+    def __init__(self, N, ncomp: int, batch: int, precision: Precision, device_index: int = 0):
+        import torch
+        self.N = tuple(int(n) for n in N)
+        self.ncomp, self.batch, self.precision = int(ncomp), int(batch), precision
+        self.device_index = int(device_index)
+        lib = _lib.lib()
+        h = C.c_void_p()
+        with torch.cuda.device(self.device_index):
+            _lib.check(lib.em3d_plan_create(*self.N, self.ncomp, self.batch, precision.code, C.byref(h)),
+                       "em3d_plan_create")
+        self.handle = h
+        kb, wb, bb = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        _lib.check(lib.em3d_plan_sizes(h, C.byref(kb), C.byref(wb), C.byref(bb)), "em3d_plan_sizes")
+        self.khat_bytes, self.work_bytes, self.build_bytes = kb.value, wb.value, bb.value
+
+    @classmethod
+    def get(cls, N, ncomp, batch, precision, device_index=0) -> "HipPlan":
+        key = (tuple(int(n) for n in N), int(ncomp), int(batch), precision, int(device_index))
+        p = cls._cache.get(key)
+        if p is None:
+            p = cls(N, ncomp, batch, precision, device_index)
+            cls._cache[key] = p
+        return p
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().em3d_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+def build_octant_spectrum(N, L, k0: float, ncomp: int, precision: Precision, device_index: int = 0):
+    """Device octant spectrum ``(K, Nx+1, Ny+1, Nz+1)``, FP64 build, plan precision."""
+    import torch
+    plan = HipPlan.get(N, ncomp, 1, precision, device_index)
+    dev = torch.device("cuda", device_index)
+    es = 8 if precision is Precision.SINGLE else 16
+    K = 6 if ncomp == 3 else 1
+    khat = torch.empty((K, N[0] + 1, N[1] + 1, N[2] + 1), dtype=precision.torch_complex_dtype, device=dev)
+    assert khat.numel() * es == plan.khat_bytes
+    scratch = torch.empty(plan.build_bytes, dtype=torch.uint8, device=dev)
+    h = [L[a] / N[a] for a in range(3)]
+    _lib.check(_lib.lib().em3d_kernel_build(plan.handle, float(k0), h[0], h[1], h[2], khat.data_ptr(),
+                                            scratch.data_ptr(), _lib.stream_handle(dev)), "em3d_kernel_build")
+    del scratch
+    return khat
+
+
+def _fold(n: int, device):
+    import torch
+    Lf = 2 * n
+    k = torch.arange(Lf, device=device)
+    flipped = k > n
+    return torch.where(flipped, Lf - k, k), flipped
+
+
+def materialize_kernel_hat(oct_khat, N, adjoint: bool = False):
+    """Full ``(3, 3, 2Nx, 2Ny, 2Nz)`` spectrum (unscaled, FFT index order) from the octant."""
+    import torch
+    dev = oct_khat.device
+    (ix, fx), (iy, fy), (iz, fz) = (_fold(N[a], dev) for a in range(3))
+    D = 8 * prod(N)
+    sx = torch.where(fx, -1.0, 1.0)[:, None, None]
+    sy = torch.where(fy, -1.0, 1.0)[None, :, None]
+    sz = torch.where(fz, -1.0, 1.0)[None, None, :]
+    signs = {(0, 1): sx * sy, (0, 2): sx * sz, (1, 2): sy * sz}
+    full = torch.empty((3, 3) + tuple(2 * n for n in N), dtype=oct_khat.dtype, device=dev)
+    for c, (a, b) in enumerate(_OCT_COMPONENTS):
+        blk = oct_khat[c][ix[:, None, None], iy[None, :, None], iz[None, None, :]] * D
+        if a != b:
+            blk = blk * signs[(a, b)].to(blk.real.dtype)
+        if adjoint:
+            blk = torch.conj(blk)
+        full[a, b] = blk
+        full[b, a] = blk
+    return full
+
+
+@dataclass(frozen=True)
+class PreparedEMKernel:
+    """Reusable spectral representation of the dyadic convolution kernel.
+
+    Holds the octant spectrum actually read by the kernels.  The adjoint is
+    always available from the same storage as ``conj(Khat(k))`` (the derived
+    adjoint); ``adjoint_storage`` records what the caller asked for and
+    ``include_adjoint=False`` disables ``rmatvec`` as in the reference.
+    ``kernel_hat`` materialises the full (3, 3, 2N, 2N, 2N) tensor on request;
+    it differs from the reference's spectrum only through the zeroed index-N
+    planes, which never reach a cropped output.
+    """
+
+    grid_shape: tuple[int, int, int]
+    grid_lengths: tuple[float, float, float]
+    wave_number: float
+    device: str
+    precision: str
+    khat_octant: object = field(repr=False)
+    adjoint_storage: AdjointStorage = "explicit"
+    build_strategy: KernelBuildStrategy = "standard"
+    device_index: int = 0
+
+    @classmethod
+    def build(cls, grid: Grid, *, k: float, include_adjoint: bool = True,
+              adjoint_storage: Literal["explicit", "derived"] = "explicit",
+              build_strategy: KernelBuildStrategy = "standard") -> "PreparedEMKernel":
+        if build_strategy not in {"standard", "streamed"}:
+            raise ValueError(f"build_strategy must be 'standard' or 'streamed', got {build_strategy!r}")
+        if adjoint_storage not in {"explicit", "derived"}:
+            raise ValueError(f"adjoint_storage must be 'explicit' or 'derived', got {adjoint_storage!r}")
+        be = grid.backend
+        require_hip(be)
+        storage: AdjointStorage = adjoint_storage if include_adjoint else "none"
+        khat = build_octant_spectrum(grid.N, grid.L, float(k), 3, be.precision, be.device_index)
+        return cls(grid_shape=tuple(int(v) for v in grid.N), grid_lengths=tuple(float(v) for v in grid.L),
+                   wave_number=float(k), device=str(be.device), precision=str(be.precision.value),
+                   khat_octant=khat, adjoint_storage=storage, build_strategy=build_strategy,
+                   device_index=be.device_index)
+
+    @property
+    def supports_adjoint(self) -> bool:
+        return self.adjoint_storage != "none"
+
+    @property
+    def persistent_nbytes(self) -> int:
+        t = self.khat_octant
+        return int(t.numel() * t.element_size())
+
+    @property
+    def kernel_hat(self):
+        return materialize_kernel_hat(self.khat_octant, self.grid_shape)
+
+    @property
+    def kernel_hat_adjoint(self):
+        if self.adjoint_storage != "explicit":
+            return None
+        return materialize_kernel_hat(self.khat_octant, self.grid_shape, adjoint=True)
+
+    def validate(self, grid: Grid, *, k: float) -> None:
+        errors: list[str] = []
+        if tuple(grid.N) != self.grid_shape:
+            errors.append(f"grid shape {grid.N!r} != {self.grid_shape!r}")
+        if not np.allclose(grid.L, self.grid_lengths, rtol=0.0, atol=0.0):
+            errors.append(f"grid lengths {grid.L!r} != {self.grid_lengths!r}")
+        if not np.isclose(float(k), self.wave_number, rtol=1e-14, atol=1e-14):
+            errors.append(f"wave number {k!r} != {self.wave_number!r}")
+        if grid.backend.device != self.device:
+            errors.append(f"device {grid.backend.device!r} != {self.device!r}")
+        if grid.backend.precision.value != self.precision:
+            errors.append(f"precision {grid.backend.precision.value!r} != {self.precision!r}")
+        if errors:
+            raise ValueError("incompatible PreparedEMKernel: " + "; ".join(errors))
+
+
+def prep_coeffs_em3d(grid: Grid, *, k: float, volume: float | None = None):
+    """Octant spectrum (the storage the HIP kernels read)."""
+    return PreparedEMKernel.build(grid, k=k, include_adjoint=False).khat_octant
+
+
+def pack_contrast(chi):
+    """Classify a (3, 3, Nx, Ny, Nz) device contrast as iso / diag / full and pack it."""
+    import torch
+    off = chi.clone()
+    for a in range(3):
+        off[a, a] = 0
+    if bool(torch.count_nonzero(off).item()):
+        return CHI_FULL, chi.contiguous()
+    d = torch.stack([chi[a, a] for a in range(3)])
+    if bool(torch.equal(d[0], d[1])) and bool(torch.equal(d[0], d[2])):
+        return CHI_ISO, d[0].contiguous()
+    return CHI_DIAG, d.contiguous()
+
+
+class _FieldApplier:
+    """Shared matvec/rmatvec plumbing: shape checks, plans and workspaces."""
+
+    def _init_applier(self, be, N, ncomp):
+        self._be = be
+        self._N = tuple(int(n) for n in N)
+        self._ncomp = ncomp
+        self._work: dict[int, object] = {}
+
+    def _field_shape(self):
+        return ((3,) if self._ncomp == 3 else ()) + self._N
+
+    def _workspace(self, plan):
+        import torch
+        w = self._work.get(plan.batch)
+        if w is None:
+            w = torch.empty(plan.work_bytes, dtype=torch.uint8, device=self._be.torch_device)
+            self._work[plan.batch] = w
+        return w
+
+    def _prepare_input(self, field):
+        import torch
+        be = self._be
+        t = field if isinstance(field, torch.Tensor) else be.array(field)
+        if t.device != be.torch_device:
+            t = t.to(be.torch_device)
+        if t.dtype != be.precision.torch_complex_dtype:
+            t = t.to(be.precision.torch_complex_dtype)
+        fs = self._field_shape()
+        if tuple(t.shape) == fs:
+            batch = 1
+        elif t.dim() == len(fs) + 1 and tuple(t.shape[1:]) == fs:
+            batch = int(t.shape[0])
+        else:
+            raise ValueError(f"field shape {tuple(t.shape)} != expected {fs} (or (batch,) + {fs})")
+        return t.contiguous(), batch
+
+    def _apply(self, field, khat, chi_kind, chi, adjoint: bool):
+        import torch
+        t, batch = self._prepare_input(field)
+        plan = HipPlan.get(self._N, self._ncomp, batch, self._be.precision, self._be.device_index)
+        y = torch.empty_like(t)
+        fn = _lib.lib().em3d_rmatvec if adjoint else _lib.lib().em3d_matvec
+        _lib.check(fn(plan.handle, khat.data_ptr(), chi.data_ptr() if chi is not None else None, chi_kind,
+                      t.data_ptr(), y.data_ptr(), self._workspace(plan).data_ptr(),
+                      _lib.stream_handle(self._be.torch_device)),
+                   "em3d_rmatvec" if adjoint else "em3d_matvec")
+        return y
+
+
+class Operator(_FieldApplier):
+    """``A u = u - B(chi u)`` on the HIP backend (operator.py:320-428)."""
+
+    def __init__(self, problem: Problem, *, prepared_kernel: PreparedEMKernel | None = None):
+        self.problem = problem
+        grid = problem.grid
+        require_hip(grid.backend)
+        if prepared_kernel is None:
+            prepared_kernel = PreparedEMKernel.build(grid, k=problem.k0)
+        else:
+            prepared_kernel.validate(grid, k=problem.k0)
+        self._prepared_kernel = prepared_kernel
+        self._init_applier(grid.backend, grid.N, 3)
+        self._chi_kind, self._chi = pack_contrast(problem.eps_tensor)
+
+    @property
+    def backend(self):
+        return self._be
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        d = 3 * prod(self._N)
+        return d, d
+
+    @property
+    def prepared_kernel(self) -> PreparedEMKernel:
+        return self._prepared_kernel
+
+    @property
+    def contrast_kind(self) -> str:
+        return {CHI_ISO: "isotropic", CHI_DIAG: "diagonal", CHI_FULL: "full"}[self._chi_kind]
+
+    def matvec(self, field):
+        return self._apply(field, self._prepared_kernel.khat_octant, self._chi_kind, self._chi, False)
+
+    def rmatvec(self, field):
+        if not self._prepared_kernel.supports_adjoint:
+            raise RuntimeError("adjoint action is unavailable: build PreparedEMKernel with include_adjoint=True")
+        return self._apply(field, self._prepared_kernel.khat_octant, self._chi_kind, self._chi, True)
+
+    def matvec_flat(self, vector):
+        return flatten_field(self.matvec(unflatten_field(vector, self._N)))
+
+    def rmatvec_flat(self, vector):
+        return flatten_field(self.rmatvec(unflatten_field(vector, self._N)))
+
+    def iteration_matvec_flat(self, vector, *, mu: complex):
+        if abs(mu) == 0.0:
+            raise ValueError("mu must be non-zero")
+        return vector - self.matvec_flat(vector) / complex(mu)
+
+    def to_dense_kernel(self):
+        raise RuntimeError("dense matrix construction requires a NumPy backend")
+
+    def to_dense_operator(self):
+        raise RuntimeError("dense matrix construction requires a NumPy backend")
+
+    def to_dense(self):
+        return self.to_dense_kernel()

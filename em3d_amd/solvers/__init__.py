@@ -1,0 +1,6 @@
+from .base import BaseSolver, SolverConfig, SolverResult
+from .bicgstab import BiCGStab
+from .sim import SIM
+from .twostep import TwoStep
+
+__all__ = ["BaseSolver", "SolverConfig", "SolverResult", "SIM", "BiCGStab", "TwoStep"]

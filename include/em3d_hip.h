@@ -1,0 +1,111 @@
+/*
+ * em3d_hip.h -- C-ABI of libem3d_hip.so, the MI355X (gfx950) implementation of
+ * the FFT volume-integral operator and Krylov vector kernels of em3d.
+ *
+ * Conventions
+ *   - Every function returns int: 0 = success, non-zero = failure with the
+ *     message in em3d_last_error() (per calling thread).  No exception crosses.
+ *   - All array pointers are DEVICE pointers owned by the caller (e.g. torch
+ *     data_ptr()); the library never frees them.  Work is enqueued on the
+ *     caller's hipStream_t and is asynchronous unless stated.
+ *   - Complex arrays are interleaved (re, im): complex64 for precision 0,
+ *     complex128 for precision 1.  Layouts are C order, z fastest:
+ *       field  u, y   (batch, ncomp, Nx, Ny, Nz)          ncomp = 3 (EM) or 1
+ *       contrast chi  CHI_ISO (Nx,Ny,Nz) | CHI_DIAG (3,Nx,Ny,Nz) |
+ *                     CHI_FULL (3,3,Nx,Ny,Nz)   (acoustic: CHI_ISO only)
+ *       khat          octant spectrum (K, Nx+1, Ny+1, Nz+1), K = 6 for EM
+ *                     (xx,yy,zz,xy,xz,yz) or 1 for acoustic, pre-scaled by
+ *                     1/(8 Nx Ny Nz)
+ *   - A plan handle must not be used by two host threads at once.
+ *
+ * Reference interfaces replaced (qwerty29544/em3d 0.8.0, paths under src/em3d):
+ * see the comment above each declaration.
+ */
+#ifndef EM3D_HIP_H
+#define EM3D_HIP_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct em3d_plan_s* em3d_plan_t;
+
+enum { EM3D_SINGLE = 0, EM3D_DOUBLE = 1 };
+enum { EM3D_CHI_NONE = 0, EM3D_CHI_ISO = 1, EM3D_CHI_DIAG = 2, EM3D_CHI_FULL = 3 };
+
+/* Library identification string. */
+const char* em3d_version(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* em3d_last_error(void);
+
+/* Plan for an (nx, ny, nz) grid, ncomp = 3 (EM, Operator) or 1 (acoustic,
+ * AcousticOperator), `batch` right-hand sides per call, precision 0/1.
+ * Holds the radix plans and twiddle tables of the three doubled axes.
+ * Replaces the per-operator state set up in Operator.__init__
+ * (operator.py:321-338) and AcousticOperator.__init__
+ * (acoustics/operator.py:28-34).  Each doubled axis 2n must factor into
+ * 2, 3, 5, 7.  Synchronous (copies the twiddle tables). */
+int em3d_plan_create(int nx, int ny, int nz, int ncomp, int batch, int precision, em3d_plan_t* plan);
+int em3d_plan_destroy(em3d_plan_t plan);
+
+/* Byte sizes of the octant spectrum, of the per-call workspace of
+ * em3d_matvec/em3d_rmatvec, and of the FP64 scratch of em3d_kernel_build. */
+int em3d_plan_sizes(em3d_plan_t plan, size_t* khat_bytes, size_t* work_bytes, size_t* build_bytes);
+
+/* Spectrum of the discrete kernel on the doubled grid, computed in FP64 on the
+ * device (spatial kernel, 3-D FFT, octant extraction) and stored in the plan
+ * precision.  EM: dyadic Green blocks with self term -I/3
+ * (PreparedEMKernel.build, operator.py:171-223; _kernel_geometry and
+ * _kernel_block_from_geometry, operator.py:28-84).  Acoustic: k0^2 dv G with
+ * the equivalent-sphere self cell (prep_coeffs_acoustic,
+ * acoustics/kernel.py:21-49).  h = cell sizes L/N per axis. */
+int em3d_kernel_build(em3d_plan_t plan, double k0, double hx, double hy, double hz, void* khat,
+                      void* build_work, hipStream_t stream);
+
+/* y = u - B(chi u) for all batch fields (Operator.matvec, operator.py:353-363;
+ * AcousticOperator.matvec, acoustics/operator.py:40-54).  `work` holds
+ * em3d_plan_sizes().work_bytes; y must not alias u. */
+int em3d_matvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
+                void* work, hipStream_t stream);
+
+/* y = v - chi^H B^H v (Operator.rmatvec with the derived adjoint
+ * conj(Khat[-k]), operator.py:365-398 and 305-317;
+ * AcousticOperator.rmatvec, acoustics/operator.py:56-67). */
+int em3d_rmatvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* v, void* y,
+                 void* work, hipStream_t stream);
+
+/* Per-launch timing of em3d_matvec/em3d_rmatvec on this plan, the measured
+ * counterpart of the reference's matvec timer (benchmark_matvec,
+ * experiments/scans.py:275-300).  While enabled, each apply call records a
+ * hipEvent before every kernel launch and after the last one and WAITS for
+ * them (one host sync per call); em3d_plan_timing returns the average
+ * milliseconds of each launch slot (z pass, y pass, x pass, y pass, z pass)
+ * over the calls since enabling.  Enabling resets the averages. */
+int em3d_plan_set_timing(em3d_plan_t plan, int enable);
+int em3d_plan_timing(em3d_plan_t plan, double* ms_per_launch, int max_launches, int* nlaunch, long long* calls);
+
+/* Bytes of the reduction scratch used by em3d_dotc. */
+size_t em3d_reduce_work_bytes(void);
+
+/* out[0..1] (device, two doubles) = sum conj(x_i) y_i accumulated in FP64
+ * (xp.vdot and xp.linalg.norm in the solvers, solvers/bicgstab.py:64-223,
+ * solvers/sim.py:48, solvers/twostep.py:99-100). */
+int em3d_dotc(int precision, long long n, const void* x, const void* y, void* out, void* red_work,
+              hipStream_t stream);
+
+/* out = a x + b y + c z (z may be NULL; out may alias x, y or z elementwise)
+ * (the array updates of solvers/bicgstab.py:145-221, sim.py:78,
+ * twostep.py:192-212). */
+int em3d_axpbypcz(int precision, long long n, double a_re, double a_im, const void* x, double b_re,
+                  double b_im, const void* y, double c_re, double c_im, const void* z, void* out,
+                  hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EM3D_HIP_H */

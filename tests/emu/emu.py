@@ -1,0 +1,106 @@
+"""ctypes wrapper of the host emulator of the HIP pass kernels (test only).
+
+``load()`` compiles tests/emu/emu_capi.cpp with g++ into a temporary
+directory (a few seconds) and returns the library.  The emulator runs the
+product's kernel phases thread by thread on host memory; see emu_capi.cpp.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(out_dir: str | None = None, extra=()) -> str:
+    out_dir = out_dir or tempfile.mkdtemp(prefix="em3d_emu_")
+    so = os.path.join(out_dir, "libem3d_emu.so")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", *extra,
+           os.path.join(HERE, "emu_capi.cpp"), "-o", so]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return so
+
+
+def load(path: str | None = None):
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    lib = C.CDLL(path or build())
+    lib.emu_plan_create.restype = C.c_void_p
+    lib.emu_plan_create.argtypes = [C.c_int] * 6
+    lib.emu_plan_destroy.argtypes = [C.c_void_p]
+    lib.emu_plan_sizes.argtypes = [C.c_void_p] + [C.POINTER(C.c_longlong)] * 3
+    lib.emu_kernel_build.argtypes = [C.c_void_p] + [C.c_double] * 4 + [C.c_void_p] * 2
+    lib.emu_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                              C.c_void_p, C.c_int]
+    lib.emu_last_error.restype = C.c_char_p
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+class EmuOperator:
+    """Emulated HIP operator: khat built by the emulated kernel build."""
+
+    def __init__(self, lib, N, L, k0, chi, ncomp, precision="double", batch=1, chi_kind=None):
+        self.lib = lib
+        self.N = tuple(int(n) for n in N)
+        self.prec = 1 if precision == "double" else 0
+        self.cdt = np.complex128 if self.prec else np.complex64
+        self.ncomp = ncomp
+        self.batch = batch
+        self.h = lib.emu_plan_create(*self.N, ncomp, batch, self.prec)
+        if not self.h:
+            raise ValueError(lib.emu_last_error().decode())
+        ke, we, be = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        lib.emu_plan_sizes(self.h, C.byref(ke), C.byref(we), C.byref(be))
+        self.khat = np.zeros(ke.value, self.cdt)
+        self.work = np.zeros(we.value, self.cdt)
+        build = np.zeros(be.value, np.complex128)
+        hx, hy, hz = (L[a] / N[a] for a in range(3))
+        lib.emu_kernel_build(self.h, float(k0), hx, hy, hz, self.khat.ctypes.data, build.ctypes.data)
+        self.chi_kind, self.chi = pack_chi(chi, ncomp, self.cdt, chi_kind)
+
+    def _apply(self, u, adjoint):
+        u = np.ascontiguousarray(u, dtype=self.cdt)
+        y = np.empty_like(u)
+        self.lib.emu_apply(self.h, self.khat.ctypes.data, self.chi.ctypes.data, self.chi_kind,
+                           u.ctypes.data, y.ctypes.data, self.work.ctypes.data, int(adjoint))
+        return y
+
+    def matvec(self, u):
+        return self._apply(u, False)
+
+    def rmatvec(self, v):
+        return self._apply(v, True)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.emu_plan_destroy(self.h)
+            self.h = None
+
+
+def pack_chi(chi, ncomp, cdt, kind=None):
+    """Pack a reference-layout contrast into the library's chi layout."""
+    chi = np.asarray(chi)
+    if ncomp == 1:
+        return 1, np.ascontiguousarray(chi, dtype=cdt)
+    if kind is None:
+        off = chi.copy()
+        for a in range(3):
+            off[a, a] = 0
+        if not np.any(off):
+            d = np.stack([chi[a, a] for a in range(3)])
+            kind = 1 if (np.array_equal(d[0], d[1]) and np.array_equal(d[0], d[2])) else 2
+        else:
+            kind = 3
+    if kind == 1:
+        return 1, np.ascontiguousarray(chi[0, 0], dtype=cdt)
+    if kind == 2:
+        return 2, np.ascontiguousarray(np.stack([chi[a, a] for a in range(3)]), dtype=cdt)
+    return 3, np.ascontiguousarray(chi, dtype=cdt)

@@ -1,0 +1,167 @@
+// Host emulator of the HIP pass kernels (TEST INFRASTRUCTURE).
+//
+// Compiles the product's block-phase headers with g++ and runs every block of
+// every pass thread by thread, phase by phase, on host memory.  It exercises
+// the exact index arithmetic, tiling, pruning and spectral folding of the GPU
+// kernels on the CPU; the LDS tile is poisoned with NaN before each block so a
+// read of an unwritten LDS element shows up in the results.
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../em3d_amd/csrc/e3_pipeline.hpp"
+
+using namespace e3;
+
+namespace {
+
+std::string g_err;
+
+template <typename T>
+void poison(std::vector<unsigned char>& mem) {
+  cpx<T>* p = (cpx<T>*)mem.data();
+  const size_t n = mem.size() / sizeof(cpx<T>);
+  const T nan = std::numeric_limits<T>::quiet_NaN();
+  for (size_t i = 0; i < n; ++i) p[i] = mk(nan, nan);
+}
+
+struct EmuLauncher {
+  int nt = kThreads;
+  template <typename T>
+  void z_t(const ZArgs& a, long long nblk, int batch, size_t lds) {
+    std::vector<unsigned char> mem(lds);
+    for (long long bid = 0; bid < nblk * batch; ++bid) {
+      poison<T>(mem);
+      const int b = (int)(bid / nblk);
+      ZPass<T> z(a, bid - (long long)b * nblk, b, (cpx<T>*)mem.data());
+      for (int t = 0; t < nt; ++t) z.load(t, nt);
+      for (int s = 0; s < a.plan.nst; ++s)
+        for (int t = 0; t < nt; ++t) z.stage(s, t, nt);
+      for (int t = 0; t < nt; ++t) z.store(t, nt);
+    }
+  }
+  void z(const ZArgs& a, long long nblk, int batch, int prec, size_t lds) {
+    if (prec == PREC_DOUBLE) z_t<double>(a, nblk, batch, lds);
+    else z_t<float>(a, nblk, batch, lds);
+  }
+  template <typename T>
+  void s_t(const SArgs& a, long long ncb, long long Q, size_t lds) {
+    std::vector<unsigned char> mem(lds);
+    for (long long bid = 0; bid < ncb * Q; ++bid) {
+      poison<T>(mem);
+      const long long q = bid / ncb;
+      SPass<T> s(a, bid - q * ncb, q, (cpx<T>*)mem.data());
+      for (int t = 0; t < nt; ++t) s.load(t, nt);
+      for (int i = 0; i < a.plan.nst; ++i)
+        for (int t = 0; t < nt; ++t) s.stage(i, t, nt);
+      for (int t = 0; t < nt; ++t) s.store(t, nt);
+    }
+  }
+  void s(const SArgs& a, long long ncb, long long Q, int prec, size_t lds) {
+    if (prec == PREC_DOUBLE) s_t<double>(a, ncb, Q, lds);
+    else s_t<float>(a, ncb, Q, lds);
+  }
+  template <typename T>
+  void x_t(const XArgs& a, long long ncb, int batch, size_t lds) {
+    std::vector<unsigned char> mem(lds);
+    for (long long bid = 0; bid < ncb * batch; ++bid) {
+      poison<T>(mem);
+      const int b = (int)(bid / ncb);
+      XPass<T> x(a, bid - (long long)b * ncb, b, (cpx<T>*)mem.data());
+      for (int t = 0; t < nt; ++t) x.load(t, nt);
+      for (int i = 0; i < a.plan.nst; ++i)
+        for (int t = 0; t < nt; ++t) x.stage_fwd(i, t, nt);
+      for (int t = 0; t < nt; ++t) x.multiply(t, nt);
+      for (int i = 0; i < a.plan.nst; ++i)
+        for (int t = 0; t < nt; ++t) x.stage_inv(i, t, nt);
+      for (int t = 0; t < nt; ++t) x.store(t, nt);
+    }
+  }
+  void x(const XArgs& a, long long ncb, int batch, int prec, size_t lds) {
+    if (prec == PREC_DOUBLE) x_t<double>(a, ncb, batch, lds);
+    else x_t<float>(a, ncb, batch, lds);
+  }
+  void kspatial(const KBArgs& a, long long n) {
+    for (long long i = 0; i < n; ++i) kernel_point(a, i);
+  }
+  void koct(const KBArgs& a, long long n, int prec) {
+    for (long long i = 0; i < n; ++i) {
+      if (prec == PREC_DOUBLE) octant_point<double>(a, i);
+      else octant_point<float>(a, i);
+    }
+  }
+};
+
+struct EmuPlan {
+  Plan p;
+  std::vector<double> twd;
+  std::vector<float> twf;
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* emu_last_error(void) { return g_err.c_str(); }
+
+void* emu_plan_create(int nx, int ny, int nz, int ncomp, int batch, int precision) {
+  EmuPlan* h = new EmuPlan();
+  g_err = plan_init(&h->p, nx, ny, nz, ncomp, batch, precision);
+  if (!g_err.empty()) {
+    delete h;
+    return nullptr;
+  }
+  size_t nf = 0;
+  for (int a = 0; a < 3; ++a) nf += (size_t)h->p.L[a];
+  h->twd.resize(2 * nf);
+  h->twf.resize(2 * nf);
+  size_t off = 0;
+  for (int a = 0; a < 3; ++a) {
+    const int L = h->p.L[a];
+    for (int t = 0; t < L; ++t) {
+      const double ang = -2.0 * M_PI * (double)t / (double)L;
+      h->twd[2 * (off + t)] = std::cos(ang);
+      h->twd[2 * (off + t) + 1] = std::sin(ang);
+      h->twf[2 * (off + t)] = (float)std::cos(ang);
+      h->twf[2 * (off + t) + 1] = (float)std::sin(ang);
+    }
+    h->p.tw_d[a] = h->twd.data() + 2 * off;
+    h->p.tw_f[a] = h->twf.data() + 2 * off;
+    off += (size_t)L;
+  }
+  return h;
+}
+
+void emu_plan_destroy(void* h) { delete (EmuPlan*)h; }
+
+void emu_plan_sizes(void* h, long long* khat_elems, long long* work_elems, long long* build_elems) {
+  const Plan& p = ((EmuPlan*)h)->p;
+  *khat_elems = p.khat_elems;
+  *work_elems = p.w1_elems + p.w2_elems;
+  *build_elems = p.build_elems;
+}
+
+void emu_kernel_build(void* h, double k0, double hx, double hy, double hz, void* khat, void* build_work) {
+  const Plan& p = ((EmuPlan*)h)->p;
+  const double hh[3] = {hx, hy, hz};
+  double sre = 0.0, sim = 0.0;
+  if (p.C == 1) {
+    const double dv = hx * hy * hz;
+    const double a = std::cbrt(3.0 * dv / (4.0 * M_PI));
+    const double ka = k0 * a;
+    sre = std::cos(ka) + std::sin(ka) * ka - 1.0;
+    sim = std::sin(ka) - std::cos(ka) * ka;
+  }
+  EmuLauncher go;
+  run_kernel_build(go, p, k0, hh, sre, sim, khat, build_work);
+}
+
+void emu_apply(void* h, const void* khat, const void* chi, int chi_kind, const void* u, void* y, void* work,
+               int adjoint) {
+  EmuLauncher go;
+  run_apply(go, ((EmuPlan*)h)->p, khat, chi, chi_kind, u, y, work, adjoint != 0);
+}
+
+}  // extern "C"

@@ -1,0 +1,223 @@
+"""GPU parity: the HIP operator and solvers against the oracle (runs on an MI355X).
+
+Tolerances (the precision contract of the build): FP64 matvec/rmatvec within
+1e-12 relative L2 of the complex128 oracle; FP32 (complex64 kernels, FP64
+kernel build) within 1e-6; solver trajectories follow the oracle's to the
+same iteration count and status.  Large sizes are checked through properties
+that do not need the oracle: adjoint identity, linearity, zero contrast,
+FP32 vs FP64 agreement.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def em():
+    import em3d_amd
+    return em3d_amd
+
+
+def _be(em, prec):
+    return em.Backend.hip(em.Precision.DOUBLE if prec == "double" else em.Precision.SINGLE)
+
+
+def _rand(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _host(t):
+    return t.detach().cpu().numpy()
+
+
+def _problem(em, be, N, L, k, chi_np):
+    grid = em.Grid(N=N, L=L, center=(0.0, 0.0, 0.0), backend=be)
+    wave = em.flat_wave_vec(grid, k=k, orient=(0, 0, 1), amplitude=(1, 0, 0))
+    return em.Problem(grid=grid, eps_tensor=be.array(chi_np.astype(be.complex_dtype)), wave=wave, k0=k,
+                      volume=grid.dv * int(np.prod(N)))
+
+
+def _chi(kind, N, seed=3):
+    rng = np.random.default_rng(seed)
+    if kind == "full":
+        return (rng.standard_normal((3, 3) + N) + 1j * rng.standard_normal((3, 3) + N)) * 0.3
+    from oracle import inputs
+    iso = inputs.ellipsis_refraction(N, (1, 1, 1), (0, 0, 0), eps_real=2.5, eps_imag=0.1, ell_center=(0, 0, 0),
+                                     radius=(0.3, 0.2, 0.15) if min(N) >= 8 else (0.45, 0.45, 0.45))
+    if kind == "iso":
+        return iso
+    d = iso.copy()
+    d[1, 1] *= (2.0 + 0.2j) / (1.5 + 0.1j)
+    d[2, 2] *= 1.0 / (1.5 + 0.1j)
+    return d
+
+
+EM_CASES = [((4, 4, 4), 1.0, "full"), ((4, 3, 2), 2.1, "full"), ((5, 6, 7), 3.0, "diag"),
+            ((8, 8, 8), 4.0, "iso"), ((16, 16, 16), 4.0, "diag"), ((12, 10, 14), 8.0, "full")]
+
+
+@pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
+@pytest.mark.parametrize("N,k,kind", EM_CASES)
+def test_em_matvec_rmatvec_vs_oracle(em, prec, tol, N, k, kind):
+    from oracle import vie
+    _say(f"[em parity] N={N} k={k} chi={kind} prec={prec}")
+    be = _be(em, prec)
+    L = (1.0, 1.0, 1.0)
+    chi = _chi(kind, N)
+    op = em.Operator(_problem(em, be, N, L, k, chi))
+    Kh = vie.kernel_hat(N, L, k)
+    u = _rand((3,) + N, 42)
+    y = _host(op.matvec(be.array(u.astype(be.complex_dtype))))
+    ya = _host(op.rmatvec(be.array(u.astype(be.complex_dtype))))
+    assert _rel(y, vie.em_matvec(chi, u, Kh)) < tol
+    assert _rel(ya, vie.em_rmatvec(chi, u, Khat=Kh)) < tol
+
+
+def test_kernel_build_matches_oracle_spectrum(em):
+    """Materialised spectrum vs FFT of the oracle's spatial kernel with the
+    index-N planes zeroed (the only difference from the reference's storage)."""
+    from oracle import vie
+    N, L, k = (6, 5, 4), (1.0, 0.9, 0.8), 3.3
+    _say(f"[kernel build] N={N}")
+    be = _be(em, "double")
+    grid = em.Grid(N=N, L=L, center=(0, 0, 0), backend=be)
+    pk = em.PreparedEMKernel.build(grid, k=k)
+    K = vie.kernel_tensor(N, L, k)
+    K[:, :, N[0], :, :] = 0
+    K[:, :, :, N[1], :] = 0
+    K[:, :, :, :, N[2]] = 0
+    ref = np.fft.fftn(K, axes=(-3, -2, -1))
+    got = _host(pk.kernel_hat)
+    assert _rel(got, ref) < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
+def test_acoustic_vs_oracle(em, prec, tol):
+    from oracle import vie
+    be = _be(em, prec)
+    for N, L, k in [((3, 3, 2), (1.0, 0.8, 0.6), 1.2), ((16, 12, 10), (1.0, 1.0, 1.0), 10.0)]:
+        _say(f"[acoustic parity] N={N} prec={prec}")
+        grid = em.Grid(N=N, L=L, center=(0, 0, 0), backend=be)
+        eta = em.acoustics.eta_sphere(grid, center=(0, 0, 0), radius=0.3, eta_inside=2.0 + 0.2j)
+        prob = em.acoustics.make_acoustic_problem(grid, eta, k0=k)
+        op = em.acoustics.AcousticOperator(prob)
+        chi = _host(prob.chi).astype(np.complex128)
+        u = _rand(N, 9)
+        y = _host(op.matvec(be.array(u.astype(be.complex_dtype))))
+        ya = _host(op.rmatvec(be.array(u.astype(be.complex_dtype))))
+        assert _rel(y, vie.acoustic_matvec(chi, u, vie.acoustic_kernel_hat(N, L, k))) < tol
+        assert _rel(ya, vie.acoustic_rmatvec(chi, u, vie.acoustic_kernel_hat(N, L, k, adjoint=True))) < tol
+
+
+def test_batched_rhs_equals_single_calls(em):
+    N, k = (16, 16, 16), 8.0
+    _say(f"[batch] N={N}")
+    be = _be(em, "single")
+    chi = _chi("diag", N)
+    op = em.Operator(_problem(em, be, N, (1, 1, 1), k, chi))
+    U = be.array(_rand((5, 3) + N, 11).astype(np.complex64))
+    Y = op.matvec(U)
+    for b in range(5):
+        assert _rel(_host(Y[b]), _host(op.matvec(U[b]))) < 1e-6
+
+
+@pytest.mark.parametrize("N", [(64, 64, 64), (128, 128, 128)])
+def test_full_size_properties(em, N):
+    """Adjoint identity, linearity, zero-contrast identity and FP32/FP64
+    agreement at the benchmark sizes (no oracle needed)."""
+    import torch
+    _say(f"[properties] N={N}")
+    k = 8.0
+    chi = _chi("diag", N)
+    be64, be32 = _be(em, "double"), _be(em, "single")
+    op64 = em.Operator(_problem(em, be64, N, (1, 1, 1), k, chi))
+    op32 = em.Operator(_problem(em, be32, N, (1, 1, 1), k, chi))
+    u = be64.array(_rand((3,) + N, 1))
+    v = be64.array(_rand((3,) + N, 2))
+    au, ahv = op64.matvec(u), op64.rmatvec(v)
+    lhs = complex(torch.vdot(au.flatten(), v.flatten()))
+    rhs = complex(torch.vdot(u.flatten(), ahv.flatten()))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+    a = 0.7 - 0.3j
+    lin = op64.matvec(a * u + v) - (a * au + op64.matvec(v))
+    assert float(torch.linalg.vector_norm(lin) / torch.linalg.vector_norm(au)) < 1e-13
+    y32 = op32.matvec(u.to(torch.complex64)).to(torch.complex128)
+    assert float(torch.linalg.vector_norm(y32 - au) / torch.linalg.vector_norm(au)) < 1e-6
+    zero = em.Operator(_problem(em, be64, N, (1, 1, 1), k, np.zeros((3, 3) + N, complex)),
+                       prepared_kernel=op64.prepared_kernel)
+    assert torch.equal(zero.matvec(u), u)
+
+
+def test_bicgstab_trajectory_matches_oracle(em):
+    """Config-1 sphere at 16^3: same iterations, status and residual history."""
+    from oracle import inputs, solvers
+    N, L, k = (16, 16, 16), (1.0, 1.0, 1.0), 4.0
+    _say("[bicgstab] 16^3 sphere")
+    mask = inputs.ellipsoid_center_mask(N, L, (0, 0, 0), ell_center=(0, 0, 0), radii=(0.25,) * 3)
+    chi = inputs.contrast_from_weights(mask.astype(float), background_eps_r=1.0, feature_eps_r=2.0)
+    wave = inputs.flat_wave_vec(N, L, (0, 0, 0), k=k, orient=(0, 0, 1), amplitude=(1, 0, 0))
+    ref = solvers.bicgstab(solvers.EMOperator(N, L, k, chi), wave, solvers.Config(max_iter=200, rtol=1e-8))
+    be = _be(em, "double")
+    grid = em.Grid(N=N, L=L, center=(0, 0, 0), backend=be)
+    prob = em.Problem(grid=grid, eps_tensor=be.array(chi), wave=be.array(wave), k0=k, volume=grid.dv * 4096)
+    op = em.Operator(prob)
+    res = em.BiCGStab(em.SolverConfig(max_iter=200, rtol=1e-8)).solve(op, prob.wave)
+    assert res.status == ref.status == "converged"
+    assert res.iterations == ref.iterations and res.matvec_count == ref.matvec_count
+    np.testing.assert_allclose(res.residual_history, ref.residual_history, rtol=1e-6)
+    assert _rel(_host(res.u), ref.u) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["SIM", "TwoStep"])
+def test_sim_twostep_match_oracle(em, name):
+    from oracle import inputs, solvers
+    N, L, k = (6, 6, 6), (1.0, 1.0, 1.0), 1.0
+    _say(f"[{name}] 6^3")
+    chi = inputs.cylinder_refraction(N, L, (0, 0, 0), eps_real=1.3, eps_imag=0.1, radius=0.3)
+    rhs = _rand((3,) + N, 5)
+    cfg = dict(max_iter=60, rtol=1e-9, mu=1.0 + 0j, radius=0.5)
+    ref = getattr(solvers, name.lower())(solvers.EMOperator(N, L, k, chi), rhs, solvers.Config(**cfg))
+    be = _be(em, "double")
+    grid = em.Grid(N=N, L=L, center=(0, 0, 0), backend=be)
+    prob = em.Problem(grid=grid, eps_tensor=be.array(chi), wave=be.array(rhs), k0=k, volume=1.0)
+    res = getattr(em, name)(em.SolverConfig(**cfg)).solve(em.Operator(prob), prob.wave)
+    assert res.status == ref.status and res.iterations == ref.iterations
+    assert res.matvec_count == ref.matvec_count and res.rmatvec_count == ref.rmatvec_count
+    np.testing.assert_allclose(res.residual_history, ref.residual_history, rtol=1e-7)
+    assert _rel(_host(res.u), ref.u) < 1e-9
+
+
+def test_forced_statuses_on_gpu(em):
+    N = (4, 4, 4)
+    be = _be(em, "double")
+    grid = em.Grid(N=N, L=(1, 1, 1), center=(0, 0, 0), backend=be)
+    chi = _chi("full", N)
+    prob = em.Problem(grid=grid, eps_tensor=be.array(chi), wave=be.array(_rand((3,) + N, 1)), k0=1.0, volume=1.0)
+    op = em.Operator(prob)
+    r = em.BiCGStab(em.SolverConfig()).solve(op, be.zeros((3,) + N, "complex"))
+    assert r.status == "converged" and r.matvec_count == 0
+    bad = _rand((3,) + N, 2)
+    bad[0, 0, 0, 0] = np.nan
+    assert em.BiCGStab(em.SolverConfig()).solve(op, be.array(bad)).status == "nonfinite"
+    r = em.BiCGStab(em.SolverConfig(max_iter=100, max_operator_actions=2)).solve(op, prob.wave)
+    assert r.status == "max_operator_actions" and r.operator_action_count <= 2
+    pk = em.PreparedEMKernel.build(grid, k=1.0, include_adjoint=False)
+    with pytest.raises(RuntimeError, match="adjoint action is unavailable"):
+        em.Operator(prob, prepared_kernel=pk).rmatvec(prob.wave)
+    with pytest.raises(ValueError, match="wave number"):
+        em.Operator(prob, prepared_kernel=em.PreparedEMKernel.build(grid, k=2.0))
