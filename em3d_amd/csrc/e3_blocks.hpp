@@ -269,6 +269,7 @@ struct XArgs {
   const void* tw;
   const void* khat;
   int adj;
+  int batch;
 };
 
 template <typename T>

@@ -14,6 +14,7 @@
 
 #include "../../include/em3d_hip.h"
 #include "e3_error.hpp"
+#include "e3_fast_launch.hpp"
 #include "e3_pipeline.hpp"
 
 using namespace e3;
@@ -114,8 +115,19 @@ struct HipLauncher {
     if (ev && launches < kMaxTimedLaunches) note(hipEventRecord(ev[launches], st));
     ++launches;
   }
+  // the fast (power-of-two) kernels take the pass when they cover it
+  bool fast_off = false;
+  template <class F, class A>
+  bool try_fast(F f32, F f64, const A& a, int prec) {
+    if (fast_off) return false;
+    hipError_t e = hipSuccess;
+    const bool done = prec == PREC_DOUBLE ? f64(a, st, &e) : f32(a, st, &e);
+    if (done) note(e);
+    return done;
+  }
   void z(const ZArgs& a, long long nblk, int batch, int prec, size_t lds) {
     mark();
+    if (try_fast(&launch_fast_z_f32, &launch_fast_z_f64, a, prec)) return;
     const dim3 g((unsigned)(nblk * batch));
     if (prec == PREC_DOUBLE) hipLaunchKernelGGL(k_zpass<double>, g, dim3(kThreads), lds, st, a, nblk);
     else hipLaunchKernelGGL(k_zpass<float>, g, dim3(kThreads), lds, st, a, nblk);
@@ -123,6 +135,7 @@ struct HipLauncher {
   }
   void s(const SArgs& a, long long ncb, long long Q, int prec, size_t lds) {
     mark();
+    if (try_fast(&launch_fast_s_f32, &launch_fast_s_f64, a, prec)) return;
     const dim3 g((unsigned)(ncb * Q));
     if (prec == PREC_DOUBLE) hipLaunchKernelGGL(k_spass<double>, g, dim3(kThreads), lds, st, a, ncb);
     else hipLaunchKernelGGL(k_spass<float>, g, dim3(kThreads), lds, st, a, ncb);
@@ -130,6 +143,7 @@ struct HipLauncher {
   }
   void x(const XArgs& a, long long ncb, int batch, int prec, size_t lds) {
     mark();
+    if (try_fast(&launch_fast_x_f32, &launch_fast_x_f64, a, prec)) return;
     const dim3 g((unsigned)(ncb * batch));
     if (prec == PREC_DOUBLE) hipLaunchKernelGGL(k_xpass<double>, g, dim3(kThreads), lds, st, a, ncb);
     else hipLaunchKernelGGL(k_xpass<float>, g, dim3(kThreads), lds, st, a, ncb);

@@ -1,0 +1,502 @@
+// Fast pass bodies for power-of-two doubled axes (16 ... 512).
+//
+// Each transform of length L = R1 * R2 is a two-stage Stockham FFT held in
+// registers: stage A (radix R1) reads straight from global memory, with the
+// zero half of the doubled axis never loaded; one LDS transpose; stage B
+// (radix R2) writes straight to global memory, with the discarded half never
+// stored.  Lanes run along the contiguous dimension of each access (the
+// column index of a strided pass, the element index of a z pass), so every
+// global access of a wavefront is a run of 128-byte segments.
+//
+// The x pass keeps the forward spectrum of all components in registers,
+// applies the 3x3 (or scalar) octant spectrum there and runs the inverse
+// transform from the same registers: the forward stage B and the inverse
+// stage A share one thread-to-butterfly mapping.
+//
+// As in e3_blocks.hpp, kernels are phase functions of (thread id, per-thread
+// state) separated by block barriers, so the CPU tests can replay them.
+#pragma once
+
+#include "e3_blocks.hpp"
+
+namespace e3 {
+
+// cos and sin of 2 pi k / 32, k = 0 .. 15
+constexpr double kCos32[16] = {
+    1.0, 0.98078528040323044913, 0.92387953251128675613, 0.83146961230254523708,
+    0.70710678118654752440, 0.55557023301960222474, 0.38268343236508977173, 0.19509032201612826785,
+    0.0, -0.19509032201612826785, -0.38268343236508977173, -0.55557023301960222474,
+    -0.70710678118654752440, -0.83146961230254523708, -0.92387953251128675613, -0.98078528040323044913};
+constexpr double kSin32[16] = {
+    0.0, 0.19509032201612826785, 0.38268343236508977173, 0.55557023301960222474,
+    0.70710678118654752440, 0.83146961230254523708, 0.92387953251128675613, 0.98078528040323044913,
+    1.0, 0.98078528040323044913, 0.92387953251128675613, 0.83146961230254523708,
+    0.70710678118654752440, 0.55557023301960222474, 0.38268343236508977173, 0.19509032201612826785};
+
+// b * exp(-/+ 2 pi i e / 32) (forward / inverse), 0 <= e < 16
+template <typename T, bool INV>
+E3_HD cpx<T> rot32(cpx<T> b, int e) {
+  if (e == 0) return b;
+  if (e == 8) return mul_mi<T, INV>(b);
+  const T c = (T)kCos32[e];
+  const T s = INV ? (T)kSin32[e] : -(T)kSin32[e];
+  return mk(b.re * c - b.im * s, b.re * s + b.im * c);
+}
+
+template <int R>
+constexpr int bit_reverse(int i) {
+  int r = 0;
+  for (int b = 1; b < R; b <<= 1) {
+    r = (r << 1) | (i & 1);
+    i >>= 1;
+  }
+  return r;
+}
+
+// In-register DFT of power-of-two size R <= 32 (iterative radix-2, DIT).
+template <typename T, int R, bool INV>
+E3_HD void fft_reg(cpx<T>* v) {
+  static_assert(R == 2 || R == 4 || R == 8 || R == 16 || R == 32, "radix must be a power of two <= 32");
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int j = bit_reverse<R>(i);
+    if (j > i) {
+      const cpx<T> t = v[i];
+      v[i] = v[j];
+      v[j] = t;
+    }
+  }
+#pragma unroll
+  for (int len = 2; len <= R; len <<= 1) {
+    const int half = len >> 1;
+#pragma unroll
+    for (int i = 0; i < R; i += len) {
+#pragma unroll
+      for (int j = 0; j < half; ++j) {
+        const cpx<T> a = v[i + j];
+        const cpx<T> b = rot32<T, INV>(v[i + j + half], j * (32 / len));
+        v[i + j] = a + b;
+        v[i + j + half] = a - b;
+      }
+    }
+  }
+}
+
+template <typename T, bool INV>
+E3_HD cpx<T> twiddle(const cpx<T>* tw, int idx, cpx<T> x) {
+  const cpx<T> w = tw[idx];
+  return x * (INV ? conj(w) : w);
+}
+
+// ---------------------------------------------------------------------------
+// Tile configuration per (precision, doubled length).  R1 = stage-A radix,
+// TC = columns per block of a strided pass, P = pencils per block of a z pass.
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+struct FastCfg {
+  static constexpr bool ok = false;
+};
+#define E3_FAST_CFG(TYPE, LEN, RAD1, COLS, PENS)        \
+  template <>                                            \
+  struct FastCfg<TYPE, LEN> {                            \
+    static constexpr bool ok = true;                     \
+    static constexpr int R1 = RAD1, TC = COLS, P = PENS; \
+  };
+E3_FAST_CFG(float, 16, 4, 64, 64)
+E3_FAST_CFG(float, 32, 8, 64, 64)
+E3_FAST_CFG(float, 64, 8, 32, 32)
+E3_FAST_CFG(float, 128, 16, 32, 32)
+E3_FAST_CFG(float, 256, 16, 16, 16)
+E3_FAST_CFG(float, 512, 32, 16, 16)
+E3_FAST_CFG(double, 16, 4, 32, 32)
+E3_FAST_CFG(double, 32, 8, 32, 32)
+E3_FAST_CFG(double, 64, 8, 16, 16)
+E3_FAST_CFG(double, 128, 16, 16, 16)
+E3_FAST_CFG(double, 256, 16, 8, 8)
+E3_FAST_CFG(double, 512, 32, 8, 8)
+#undef E3_FAST_CFG
+
+// ---------------------------------------------------------------------------
+// Strided pass: (Q planes) x (L rows, live n_in) x (W columns), row stride W.
+// PIN: only the first L/2 rows are live; POUT: only the first L/2 rows kept.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int R1_, int TC_, bool INV, bool PIN, bool POUT>
+struct FastS {
+  using Real = T;
+  static constexpr int R1 = R1_, R2 = L / R1_, TC = TC_, NT = TC * R2, TCP = TC + 1;
+  static constexpr int NB = R1 / R2;          // stage-B butterflies per thread
+  static constexpr int LDS = L * TCP;         // LDS elements
+  static constexpr int MA = PIN ? R1 / 2 : R1;
+  static constexpr int MB = POUT ? R2 / 2 : R2;
+  static_assert(R1 % R2 == 0, "R2 must divide R1");
+  struct St {
+    cpx<T> v[R1];
+  };
+  const SArgs& a;
+  long long q, w0;
+  cpx<T>* lds;
+  E3_HD FastS(const SArgs& args, long long colblk, long long plane, cpx<T>* l)
+      : a(args), q(plane), w0(colblk * TC), lds(l) {}
+
+  E3_HD void stage_a(int tid, St& s) const {
+    const int col = tid % TC, j = tid / TC;
+    const long long w = w0 + col;
+    const bool ok = w < a.W;
+    const cpx<T>* in = (const cpx<T>*)a.in + (size_t)q * a.in_qs;
+#pragma unroll
+    for (int m = 0; m < R1; ++m)
+      s.v[m] = (m < MA && ok) ? in[(size_t)(j + m * R2) * a.W + w] : czero<T>();
+    fft_reg<T, R1, INV>(s.v);
+#pragma unroll
+    for (int k = 0; k < R1; ++k) lds[(k * R2 + j) * TCP + col] = s.v[k];
+  }
+  E3_HD void stage_b(int tid, St& s) const {
+    const int col = tid % TC, j = tid / TC;
+    const long long w = w0 + col;
+    const cpx<T>* tw = (const cpx<T>*)a.tw;
+    cpx<T>* out = (cpx<T>*)a.out + (size_t)q * a.out_qs;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int jp = j + b * R2;
+      cpx<T>* v = s.v + b * R2;
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const cpx<T> x = lds[(jp * R2 + r) * TCP + col];
+        v[r] = r ? twiddle<T, INV>(tw, jp * r, x) : x;
+      }
+      fft_reg<T, R2, INV>(v);
+      if (w < a.W) {
+#pragma unroll
+        for (int m = 0; m < MB; ++m) out[(size_t)(jp + m * R1) * a.W + w] = v[m];
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// x pass: forward (Nx = L/2 live rows), octant spectral multiply, inverse,
+// Nx rows kept, in place on W2 viewed per (b, c) as (Nx, W = Ly Lz).
+// ---------------------------------------------------------------------------
+template <typename T, int L, int R1_, int TC_, int C>
+struct FastX {
+  using Real = T;
+  static constexpr int R1 = R1_, R2 = L / R1_, TC = TC_, NT = TC * R2, TCP = TC + 1;
+  static constexpr int NB = R1 / R2;
+  static constexpr int LDS = L * TCP;
+  struct St {
+    cpx<T> X[C][R1];   // forward spectrum: X[c][b*R2 + m] at kx = (j + b R2) + m R1
+  };
+  const XArgs& a;
+  long long w0;
+  int b;
+  cpx<T>* lds;
+  E3_HD FastX(const XArgs& args, long long colblk, int bidx, cpx<T>* l)
+      : a(args), w0(colblk * TC), b(bidx), lds(l) {}
+
+  E3_HD cpx<T>* plane(int c) const { return (cpx<T>*)a.data + (size_t)b * a.bs + (size_t)c * a.cs; }
+
+  E3_HD void fwd_a(int c, int tid) const {
+    const int col = tid % TC, j = tid / TC;
+    const long long w = w0 + col;
+    const bool ok = w < a.W;
+    const cpx<T>* d = plane(c);
+    cpx<T> v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) v[m] = (m < R1 / 2 && ok) ? d[(size_t)(j + m * R2) * a.W + w] : czero<T>();
+    fft_reg<T, R1, false>(v);
+#pragma unroll
+    for (int k = 0; k < R1; ++k) lds[(k * R2 + j) * TCP + col] = v[k];
+  }
+  E3_HD void fwd_b(int c, int tid, St& s) const {
+    const int col = tid % TC, j = tid / TC;
+    const cpx<T>* tw = (const cpx<T>*)a.tw;
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+      const int jp = j + bb * R2;
+      cpx<T>* v = s.X[c] + bb * R2;
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const cpx<T> x = lds[(jp * R2 + r) * TCP + col];
+        v[r] = r ? twiddle<T, false>(tw, jp * r, x) : x;
+      }
+      fft_reg<T, R2, false>(v);
+    }
+  }
+  E3_HD void multiply(int tid, St& s) const {
+    const int col = tid % TC, j = tid / TC;
+    const long long w = w0 + col;
+    if (w >= a.W) return;
+    const int ky = (int)(w / a.L[2]), kz = (int)(w - (long long)ky * a.L[2]);
+    int N[3] = {a.N[0], a.N[1], a.N[2]}, Ld[3] = {a.L[0], a.L[1], a.L[2]};
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+#pragma unroll
+      for (int m = 0; m < R2; ++m) {
+        const int kx = j + bb * R2 + m * R1;
+        cpx<T> f[3];
+#pragma unroll
+        for (int c = 0; c < C; ++c) f[c] = s.X[c][bb * R2 + m];
+        spectral_multiply<T>((const cpx<T>*)a.khat, C, N, Ld, kx, ky, kz, a.adj != 0, f);
+#pragma unroll
+        for (int c = 0; c < C; ++c) s.X[c][bb * R2 + m] = f[c];
+      }
+    }
+  }
+  E3_HD void inv_a(int c, int tid, St& s) const {
+    const int col = tid % TC, j = tid / TC;
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+      const int jp = j + bb * R2;
+      cpx<T>* v = s.X[c] + bb * R2;
+      fft_reg<T, R2, true>(v);
+#pragma unroll
+      for (int m2 = 0; m2 < R2; ++m2) lds[(m2 * R1 + jp) * TCP + col] = v[m2];
+    }
+  }
+  E3_HD void inv_b(int c, int tid) const {
+    const int col = tid % TC, j = tid / TC;
+    const long long w = w0 + col;
+    const cpx<T>* tw = (const cpx<T>*)a.tw;
+    cpx<T> v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) {
+      const cpx<T> x = lds[(j * R1 + m) * TCP + col];
+      v[m] = m ? twiddle<T, true>(tw, j * m, x) : x;
+    }
+    fft_reg<T, R1, true>(v);
+    if (w < a.W) {
+      cpx<T>* d = plane(c);
+#pragma unroll
+      for (int m = 0; m < R1 / 2; ++m) d[(size_t)(j + m * R2) * a.W + w] = v[m];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// z pass, forward: pencils of the contiguous axis, components one after the
+// other.  PIN: n_in = L/2 with the contrast prologue; else a full transform.
+// ---------------------------------------------------------------------------
+template <typename T>
+E3_HD cpx<T> contrast_row(const ZArgs& a, const cpx<T>* in, int c, long long p, int n) {
+  const cpx<T> uc = in[(size_t)c * a.in_cs + (size_t)p * a.in_ps + n];
+  if (a.pro != PRO_CHI || a.chi_kind == CHI_NONE) return uc;
+  const cpx<T>* chi = (const cpx<T>*)a.chi;
+  const long long cell = p * a.n_in + n;
+  if (a.chi_kind == CHI_ISO) return chi[cell] * uc;
+  if (a.chi_kind == CHI_DIAG) return chi[(size_t)c * a.ncell + cell] * uc;
+  cpx<T> acc = czero<T>();
+  for (int k = 0; k < 3; ++k)
+    acc = acc + chi[(size_t)(c * 3 + k) * a.ncell + cell] * in[(size_t)k * a.in_cs + (size_t)p * a.in_ps + n];
+  return acc;
+}
+
+template <typename T, int L, int R1_, int P_, bool PIN>
+struct FastZF {
+  using Real = T;
+  static constexpr int R1 = R1_, R2 = L / R1_, P = P_, NT = P * R2, R2P = R2 + 1;
+  static constexpr int PS = R1 * R2P;          // LDS elements per pencil
+  static constexpr int NB = R1 / R2;
+  static constexpr int LDS = P * PS;
+  static constexpr int MA = PIN ? R1 / 2 : R1;
+  struct St {
+    int unused;
+  };
+  const ZArgs& a;
+  long long p0;
+  int b;
+  cpx<T>* lds;
+  E3_HD FastZF(const ZArgs& args, long long blk, int bidx, cpx<T>* l) : a(args), p0(blk * P), b(bidx), lds(l) {}
+
+  E3_HD void stage_a(int c, int tid) const {
+    const int j = tid % R2, pl = tid / R2;
+    const long long p = p0 + pl;
+    const bool ok = p < a.P;
+    const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs;
+    cpx<T> v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) v[m] = (m < MA && ok) ? contrast_row<T>(a, in, c, p, j + m * R2) : czero<T>();
+    fft_reg<T, R1, false>(v);
+#pragma unroll
+    for (int k = 0; k < R1; ++k) lds[pl * PS + k * R2P + j] = v[k];
+  }
+  E3_HD void stage_b(int c, int tid) const {
+    const int j = tid % R2, pl = tid / R2;
+    const long long p = p0 + pl;
+    const cpx<T>* tw = (const cpx<T>*)a.tw;
+    cpx<T>* out = (cpx<T>*)a.out + (size_t)b * a.out_bs + (size_t)c * a.out_cs + (size_t)p * a.out_ps;
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+      const int jp = j + bb * R2;
+      cpx<T> v[R2];
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const cpx<T> x = lds[pl * PS + jp * R2P + r];
+        v[r] = r ? twiddle<T, false>(tw, jp * r, x) : x;
+      }
+      fft_reg<T, R2, false>(v);
+      if (p < a.P) {
+#pragma unroll
+        for (int m = 0; m < R2; ++m) out[jp + m * R1] = v[m];
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// z pass, inverse with the epilogue y = u - t or y = v - chi^H t; keeps the
+// first L/2 outputs of all C components in registers until the epilogue.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int R1_, int P_, int C>
+struct FastZI {
+  using Real = T;
+  static constexpr int R1 = R1_, R2 = L / R1_, P = P_, NT = P * R2, R2P = R2 + 1;
+  static constexpr int PS = R1 * R2P;
+  static constexpr int NB = R1 / R2;
+  static constexpr int LDS = P * PS;
+  static constexpr int MH = R2 / 2;
+  struct St {
+    cpx<T> y[C][NB][MH];
+  };
+  const ZArgs& a;
+  long long p0;
+  int b;
+  cpx<T>* lds;
+  E3_HD FastZI(const ZArgs& args, long long blk, int bidx, cpx<T>* l) : a(args), p0(blk * P), b(bidx), lds(l) {}
+
+  E3_HD void stage_a(int c, int tid) const {
+    const int j = tid % R2, pl = tid / R2;
+    const long long p = p0 + pl;
+    const bool ok = p < a.P;
+    const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs + (size_t)c * a.in_cs + (size_t)p * a.in_ps;
+    cpx<T> v[R1];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) v[m] = ok ? in[j + m * R2] : czero<T>();
+    fft_reg<T, R1, true>(v);
+#pragma unroll
+    for (int k = 0; k < R1; ++k) lds[pl * PS + k * R2P + j] = v[k];
+  }
+  E3_HD void stage_b(int c, int tid, St& s) const {
+    const int j = tid % R2, pl = tid / R2;
+    const cpx<T>* tw = (const cpx<T>*)a.tw;
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+      const int jp = j + bb * R2;
+      cpx<T> v[R2];
+#pragma unroll
+      for (int r = 0; r < R2; ++r) {
+        const cpx<T> x = lds[pl * PS + jp * R2P + r];
+        v[r] = r ? twiddle<T, true>(tw, jp * r, x) : x;
+      }
+      fft_reg<T, R2, true>(v);
+#pragma unroll
+      for (int m = 0; m < MH; ++m) s.y[c][bb][m] = v[m];
+    }
+  }
+  E3_HD void epilogue(int tid, St& s) const {
+    const int j = tid % R2, pl = tid / R2;
+    const long long p = p0 + pl;
+    if (p >= a.P) return;
+    cpx<T>* out = (cpx<T>*)a.out + (size_t)b * a.out_bs;
+    const cpx<T>* src = (const cpx<T>*)a.u + (size_t)b * a.out_bs;
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+#pragma unroll
+      for (int m = 0; m < MH; ++m) {
+        const int z = j + bb * R2 + m * R1;
+        cpx<T> t[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) t[c] = s.y[c][bb][m];
+        if (a.epi == EPI_SUB_CHIH) {
+          cpx<T> wv[3];
+          chi_apply<T>((const cpx<T>*)a.chi, a.chi_kind, a.ncell, p * a.n_out + z, C, t, wv, true);
+#pragma unroll
+          for (int c = 0; c < C; ++c) t[c] = wv[c];
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const size_t o = (size_t)c * a.out_cs + (size_t)p * a.out_ps + z;
+          out[o] = src[o] - t[c];
+        }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Dispatch: map a generic pass description onto a fast instantiation.  The
+// Runner supplies  template <class K> void s(const SArgs&), x(const XArgs&),
+// zf<K>(const ZArgs&), zi<K>(const ZArgs&).  Returns false when no fast
+// kernel covers the pass (the caller then uses the generic kernels).
+// ---------------------------------------------------------------------------
+template <typename T, int L, class Run>
+bool fast_s_len(Run& run, const SArgs& a) {
+  using Cfg = FastCfg<T, L>;
+  if (a.n_in * 2 == L && a.n_out == L && !a.inv) {
+    run.template s<FastS<T, L, Cfg::R1, Cfg::TC, false, true, false>>(a);
+    return true;
+  }
+  if (a.n_in == L && a.n_out * 2 == L && a.inv) {
+    run.template s<FastS<T, L, Cfg::R1, Cfg::TC, true, false, true>>(a);
+    return true;
+  }
+  if constexpr (sizeof(T) == 8) {
+    if (a.n_in == L && a.n_out == L && !a.inv) {
+      run.template s<FastS<T, L, Cfg::R1, Cfg::TC, false, false, false>>(a);
+      return true;
+    }
+  }
+  return false;
+}
+
+template <typename T, int L, class Run>
+bool fast_x_len(Run& run, const XArgs& a) {
+  using Cfg = FastCfg<T, L>;
+  if (a.N[0] * 2 != L) return false;
+  if (a.C == 3) run.template x<FastX<T, L, Cfg::R1, Cfg::TC, 3>>(a);
+  else run.template x<FastX<T, L, Cfg::R1, Cfg::TC, 1>>(a);
+  return true;
+}
+
+template <typename T, int L, class Run>
+bool fast_z_len(Run& run, const ZArgs& a) {
+  using Cfg = FastCfg<T, L>;
+  if (a.C != 1 && a.C != 3) return false;
+  if (a.n_in * 2 == L && a.n_out == L && a.inv == 0 && a.epi == EPI_STORE) {
+    run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, true>>(a);
+    return true;
+  }
+  if (a.n_in == L && a.n_out * 2 == L && a.inv == 1 && a.pro == PRO_COPY &&
+      (a.epi == EPI_SUB || a.epi == EPI_SUB_CHIH)) {
+    if (a.C == 3) run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 3>>(a);
+    else run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 1>>(a);
+    return true;
+  }
+  if constexpr (sizeof(T) == 8) {
+    if (a.n_in == L && a.n_out == L && a.inv == 0 && a.pro == PRO_COPY && a.epi == EPI_STORE) {
+      run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, false>>(a);
+      return true;
+    }
+  }
+  return false;
+}
+
+#define E3_FAST_SWITCH(FN, T, RUN, ARGS)        \
+  switch (ARGS.plan.L) {                        \
+    case 16: return FN<T, 16>(RUN, ARGS);       \
+    case 32: return FN<T, 32>(RUN, ARGS);       \
+    case 64: return FN<T, 64>(RUN, ARGS);       \
+    case 128: return FN<T, 128>(RUN, ARGS);     \
+    case 256: return FN<T, 256>(RUN, ARGS);     \
+    case 512: return FN<T, 512>(RUN, ARGS);     \
+    default: return false;                      \
+  }
+
+template <typename T, class Run>
+bool fast_s(Run& run, const SArgs& a) { E3_FAST_SWITCH(fast_s_len, T, run, a) }
+template <typename T, class Run>
+bool fast_x(Run& run, const XArgs& a) { E3_FAST_SWITCH(fast_x_len, T, run, a) }
+template <typename T, class Run>
+bool fast_z(Run& run, const ZArgs& a) { E3_FAST_SWITCH(fast_z_len, T, run, a) }
+#undef E3_FAST_SWITCH
+
+}  // namespace e3

@@ -1,0 +1,131 @@
+// HIP kernels and launch helpers of the fast (power-of-two) passes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "e3_fast.hpp"
+
+namespace e3 {
+
+template <class K>
+__global__ __launch_bounds__(K::NT) void k_fast_s(SArgs a, long long ncb) {
+  using T = typename K::Real;
+  __shared__ cpx<T> lds[K::LDS];
+  const long long bid = blockIdx.x;
+  const long long q = bid / ncb;
+  K k(a, bid - q * ncb, q, lds);
+  typename K::St st;
+  k.stage_a(threadIdx.x, st);
+  __syncthreads();
+  k.stage_b(threadIdx.x, st);
+}
+
+template <class K, int C>
+__global__ __launch_bounds__(K::NT) void k_fast_x(XArgs a, long long ncb) {
+  using T = typename K::Real;
+  __shared__ cpx<T> lds[K::LDS];
+  const long long bid = blockIdx.x;
+  const int b = (int)(bid / ncb);
+  K k(a, bid - (long long)b * ncb, b, lds);
+  typename K::St st;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    k.fwd_a(c, threadIdx.x);
+    __syncthreads();
+    k.fwd_b(c, threadIdx.x, st);
+    __syncthreads();
+  }
+  k.multiply(threadIdx.x, st);
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    k.inv_a(c, threadIdx.x, st);
+    __syncthreads();
+    k.inv_b(c, threadIdx.x);
+    if (c + 1 < C) __syncthreads();
+  }
+}
+
+template <class K>
+__global__ __launch_bounds__(K::NT) void k_fast_zf(ZArgs a, long long nblk) {
+  using T = typename K::Real;
+  __shared__ cpx<T> lds[K::LDS];
+  const long long bid = blockIdx.x;
+  const int b = (int)(bid / nblk);
+  K k(a, bid - (long long)b * nblk, b, lds);
+  for (int c = 0; c < a.C; ++c) {
+    k.stage_a(c, threadIdx.x);
+    __syncthreads();
+    k.stage_b(c, threadIdx.x);
+    if (c + 1 < a.C) __syncthreads();
+  }
+}
+
+template <class K, int C>
+__global__ __launch_bounds__(K::NT) void k_fast_zi(ZArgs a, long long nblk) {
+  using T = typename K::Real;
+  __shared__ cpx<T> lds[K::LDS];
+  const long long bid = blockIdx.x;
+  const int b = (int)(bid / nblk);
+  K k(a, bid - (long long)b * nblk, b, lds);
+  typename K::St st;
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    k.stage_a(c, threadIdx.x);
+    __syncthreads();
+    k.stage_b(c, threadIdx.x, st);
+    if (c + 1 < C) __syncthreads();
+  }
+  k.epilogue(threadIdx.x, st);
+}
+
+template <class X>
+struct CompOf;
+template <typename T, int L, int R1, int TC, int C>
+struct CompOf<FastX<T, L, R1, TC, C>> {
+  static constexpr int value = C;
+};
+template <typename T, int L, int R1, int P, int C>
+struct CompOf<FastZI<T, L, R1, P, C>> {
+  static constexpr int value = C;
+};
+
+struct HipFastRun {
+  hipStream_t st;
+  hipError_t err = hipSuccess;
+  template <class K>
+  void s(const SArgs& a) {
+    const long long ncb = (a.W + K::TC - 1) / K::TC;
+    hipLaunchKernelGGL(k_fast_s<K>, dim3((unsigned)(ncb * a.Q)), dim3(K::NT), 0, st, a, ncb);
+    err = hipGetLastError();
+  }
+  template <class K>
+  void x(const XArgs& a) {
+    const long long ncb = (a.W + K::TC - 1) / K::TC;
+    hipLaunchKernelGGL((k_fast_x<K, CompOf<K>::value>), dim3((unsigned)(ncb * a.batch)), dim3(K::NT), 0, st,
+                       a, ncb);
+    err = hipGetLastError();
+  }
+  template <class K>
+  void zf(const ZArgs& a) {
+    const long long nblk = (a.P + K::P - 1) / K::P;
+    hipLaunchKernelGGL(k_fast_zf<K>, dim3((unsigned)(nblk * a.batch)), dim3(K::NT), 0, st, a, nblk);
+    err = hipGetLastError();
+  }
+  template <class K>
+  void zi(const ZArgs& a) {
+    const long long nblk = (a.P + K::P - 1) / K::P;
+    hipLaunchKernelGGL((k_fast_zi<K, CompOf<K>::value>), dim3((unsigned)(nblk * a.batch)), dim3(K::NT), 0, st,
+                       a, nblk);
+    err = hipGetLastError();
+  }
+};
+
+// Defined in e3_fast_f32.hip / e3_fast_f64.hip.
+bool launch_fast_s_f32(const SArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_x_f32(const XArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_z_f32(const ZArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_s_f64(const SArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_x_f64(const XArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_z_f64(const ZArgs& a, hipStream_t st, hipError_t* err);
+
+}  // namespace e3

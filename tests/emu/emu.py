@@ -39,6 +39,7 @@ def load(path: str | None = None):
     lib.emu_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                               C.c_void_p, C.c_int]
     lib.emu_last_error.restype = C.c_char_p
+    lib.emu_set_fast.argtypes = [C.c_int]
     if path is None:
         _LIB = lib
     return lib

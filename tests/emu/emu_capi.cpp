@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "../../em3d_amd/csrc/e3_fast.hpp"
 #include "../../em3d_amd/csrc/e3_pipeline.hpp"
 
 using namespace e3;
@@ -27,8 +28,86 @@ void poison(std::vector<unsigned char>& mem) {
   for (size_t i = 0; i < n; ++i) p[i] = mk(nan, nan);
 }
 
+// Replays the fast (power-of-two) kernels: per-thread register state lives
+// in an array indexed by thread id; barriers separate the phase loops.
+struct EmuFastRun {
+  template <class K>
+  static std::vector<unsigned char> tile() {
+    std::vector<unsigned char> m((size_t)K::LDS * sizeof(cpx<typename K::Real>));
+    poison<typename K::Real>(m);
+    return m;
+  }
+  template <class K>
+  void s(const SArgs& a) {
+    using T = typename K::Real;
+    const long long ncb = (a.W + K::TC - 1) / K::TC;
+    for (long long bid = 0; bid < ncb * a.Q; ++bid) {
+      auto mem = tile<K>();
+      const long long q = bid / ncb;
+      K k(a, bid - q * ncb, q, (cpx<T>*)mem.data());
+      std::vector<typename K::St> st(K::NT);
+      for (int t = 0; t < K::NT; ++t) k.stage_a(t, st[t]);
+      for (int t = 0; t < K::NT; ++t) k.stage_b(t, st[t]);
+    }
+  }
+  template <class K>
+  void x(const XArgs& a) {
+    using T = typename K::Real;
+    const long long ncb = (a.W + K::TC - 1) / K::TC;
+    const int C = a.C;
+    for (long long bid = 0; bid < ncb * a.batch; ++bid) {
+      auto mem = tile<K>();
+      const int b = (int)(bid / ncb);
+      K k(a, bid - (long long)b * ncb, b, (cpx<T>*)mem.data());
+      std::vector<typename K::St> st(K::NT);
+      for (int c = 0; c < C; ++c) {
+        for (int t = 0; t < K::NT; ++t) k.fwd_a(c, t);
+        for (int t = 0; t < K::NT; ++t) k.fwd_b(c, t, st[t]);
+      }
+      for (int t = 0; t < K::NT; ++t) k.multiply(t, st[t]);
+      for (int c = 0; c < C; ++c) {
+        for (int t = 0; t < K::NT; ++t) k.inv_a(c, t, st[t]);
+        for (int t = 0; t < K::NT; ++t) k.inv_b(c, t);
+      }
+    }
+  }
+  template <class K>
+  void zf(const ZArgs& a) {
+    using T = typename K::Real;
+    const long long nblk = (a.P + K::P - 1) / K::P;
+    for (long long bid = 0; bid < nblk * a.batch; ++bid) {
+      auto mem = tile<K>();
+      const int b = (int)(bid / nblk);
+      K k(a, bid - (long long)b * nblk, b, (cpx<T>*)mem.data());
+      for (int c = 0; c < a.C; ++c) {
+        for (int t = 0; t < K::NT; ++t) k.stage_a(c, t);
+        for (int t = 0; t < K::NT; ++t) k.stage_b(c, t);
+      }
+    }
+  }
+  template <class K>
+  void zi(const ZArgs& a) {
+    using T = typename K::Real;
+    const long long nblk = (a.P + K::P - 1) / K::P;
+    for (long long bid = 0; bid < nblk * a.batch; ++bid) {
+      auto mem = tile<K>();
+      const int b = (int)(bid / nblk);
+      K k(a, bid - (long long)b * nblk, b, (cpx<T>*)mem.data());
+      std::vector<typename K::St> st(K::NT);
+      for (int c = 0; c < a.C; ++c) {
+        for (int t = 0; t < K::NT; ++t) k.stage_a(c, t);
+        for (int t = 0; t < K::NT; ++t) k.stage_b(c, t, st[t]);
+      }
+      for (int t = 0; t < K::NT; ++t) k.epilogue(t, st[t]);
+    }
+  }
+};
+
+int g_use_fast = 1;
+
 struct EmuLauncher {
   int nt = kThreads;
+  EmuFastRun fr;
   template <typename T>
   void z_t(const ZArgs& a, long long nblk, int batch, size_t lds) {
     std::vector<unsigned char> mem(lds);
@@ -43,6 +122,7 @@ struct EmuLauncher {
     }
   }
   void z(const ZArgs& a, long long nblk, int batch, int prec, size_t lds) {
+    if (g_use_fast && (prec == PREC_DOUBLE ? fast_z<double>(fr, a) : fast_z<float>(fr, a))) return;
     if (prec == PREC_DOUBLE) z_t<double>(a, nblk, batch, lds);
     else z_t<float>(a, nblk, batch, lds);
   }
@@ -60,6 +140,7 @@ struct EmuLauncher {
     }
   }
   void s(const SArgs& a, long long ncb, long long Q, int prec, size_t lds) {
+    if (g_use_fast && (prec == PREC_DOUBLE ? fast_s<double>(fr, a) : fast_s<float>(fr, a))) return;
     if (prec == PREC_DOUBLE) s_t<double>(a, ncb, Q, lds);
     else s_t<float>(a, ncb, Q, lds);
   }
@@ -80,6 +161,7 @@ struct EmuLauncher {
     }
   }
   void x(const XArgs& a, long long ncb, int batch, int prec, size_t lds) {
+    if (g_use_fast && (prec == PREC_DOUBLE ? fast_x<double>(fr, a) : fast_x<float>(fr, a))) return;
     if (prec == PREC_DOUBLE) x_t<double>(a, ncb, batch, lds);
     else x_t<float>(a, ncb, batch, lds);
   }
@@ -105,6 +187,8 @@ struct EmuPlan {
 extern "C" {
 
 const char* emu_last_error(void) { return g_err.c_str(); }
+
+void emu_set_fast(int on) { g_use_fast = on; }
 
 void* emu_plan_create(int nx, int ny, int nz, int ncomp, int batch, int precision) {
   EmuPlan* h = new EmuPlan();

@@ -108,3 +108,41 @@ print("ok")
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
     r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, env=env, timeout=300)
     assert r.returncode == 0 and "ok" in r.stdout, r.stderr[-3000:]
+
+
+FAST_CASES = [((8, 8, 8), (1.0, 1.0, 1.0), 4.0), ((16, 8, 4), (1.0, 0.8, 0.6), 2.0),
+              ((8, 16, 32), (1.0, 1.0, 1.0), 3.0), ((5, 8, 16), (1.0, 1.0, 1.0), 3.0)]
+
+
+@pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("single", 1e-6)])
+@pytest.mark.parametrize("N,L,k", FAST_CASES)
+def test_fast_power_of_two_kernels_match_oracle(lib, N, L, k, prec, tol):
+    """Power-of-two doubled axes take the register-resident two-stage kernels
+    (e3_fast.hpp); mixed shapes combine them with the generic passes."""
+    lib.emu_set_fast(1)
+    chi = _rand((3, 3) + N, 8) * 0.3
+    u = _rand((2, 3) + N, 9)
+    op = emu.EmuOperator(lib, N, L, k, chi, 3, precision=prec, batch=2)
+    Kh = vie.kernel_hat(N, L, k)
+    y, ya = op.matvec(u), op.rmatvec(u)
+    for b in range(2):
+        assert _rel(y[b], vie.em_matvec(chi, u[b], Kh)) < tol
+        assert _rel(ya[b], vie.em_rmatvec(chi, u[b], Khat=Kh)) < tol
+    ca = _rand(N, 10) * 0.3
+    ua = _rand(N, 11)
+    opa = emu.EmuOperator(lib, N, L, k, ca, 1, precision=prec)
+    assert _rel(opa.matvec(ua), vie.acoustic_matvec(ca, ua, vie.acoustic_kernel_hat(N, L, k))) < tol
+    assert _rel(opa.rmatvec(ua), vie.acoustic_rmatvec(ca, ua, vie.acoustic_kernel_hat(N, L, k, adjoint=True))) < tol
+
+
+def test_generic_and_fast_paths_agree(lib):
+    N, L, k = (8, 8, 8), (1.0, 1.0, 1.0), 5.0
+    chi = _rand((3, 3) + N, 12) * 0.3
+    u = _rand((3,) + N, 13)
+    lib.emu_set_fast(0)
+    try:
+        slow = emu.EmuOperator(lib, N, L, k, chi, 3).matvec(u)
+    finally:
+        lib.emu_set_fast(1)
+    fast = emu.EmuOperator(lib, N, L, k, chi, 3).matvec(u)
+    assert _rel(fast, slow) < 1e-14
