@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--k0", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run 3 plain device copies of 512 MiB (read 512 MiB, write 512 MiB each), "
+                         "the calibration kernel for HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -138,6 +141,15 @@ def main():
     achieved = pb[dom] / (per_launch[dom] * 1e-3) / 1e9
     total_bytes = sum(pb)
     log(f"[bench] per-launch ms {[round(x, 4) for x in per_launch]}")
+
+    if args.copy_probe:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.complex64, device=be32.torch_device)
+        dst = torch.empty_like(src)
+        src.real.fill_(1.0)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     solve = None
     if not args.no_solve:

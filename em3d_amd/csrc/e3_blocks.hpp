@@ -124,6 +124,24 @@ E3_HD void chi_apply(const cpx<T>* chi, int kind, long long ncell, long long cel
 }
 
 // ---------------------------------------------------------------------------
+// Mirror-interleaved order of a doubled axis (length L = 2N) in the
+// intermediates W1/W2: position 0 <- k = 0, 1 <- k = N, 2i <- k = i and
+// 2i + 1 <- k = L - i (0 < i < N).  Frequencies k and L - k, which share one
+// octant entry of the spectrum, become neighbours, so a 16-wide column tile of
+// the x pass reads 8 distinct octant entries instead of 16.
+// ---------------------------------------------------------------------------
+E3_HD int mirror_position(int k, int N, int L) {
+  if (k == 0) return 0;
+  if (k == N) return 1;
+  return k < N ? 2 * k : 2 * (L - k) + 1;
+}
+E3_HD int mirror_frequency(int pos, int N, int L) {
+  if (pos == 0) return 0;
+  if (pos == 1) return N;
+  return (pos & 1) ? L - (pos >> 1) : (pos >> 1);
+}
+
+// ---------------------------------------------------------------------------
 // Z pass: transform along the contiguous axis, C components per pencil.
 // ---------------------------------------------------------------------------
 struct ZArgs {
@@ -146,6 +164,8 @@ struct ZArgs {
   long long ncell;
   const void* u;               // epilogue source, same layout as out
   int batch;
+  int perm;                    // doubled axis in mirror-interleaved order (output of a
+                               // forward pass, input of an inverse pass)
 };
 
 template <typename T>
@@ -167,7 +187,8 @@ struct ZPass {
       cpx<T> w[6];
       if (p < a.P && z < a.n_in) {
         cpx<T> v[6];
-        for (int c = 0; c < a.C; ++c) v[c] = in[c * a.in_cs + p * a.in_ps + z];
+        const int zi = (a.perm && a.inv) ? mirror_position(z, a.n_in / 2, a.n_in) : z;
+        for (int c = 0; c < a.C; ++c) v[c] = in[c * a.in_cs + p * a.in_ps + zi];
         if (a.pro == PRO_CHI)
           chi_apply<T>((const cpx<T>*)a.chi, a.chi_kind, a.ncell, p * a.n_in + z, a.C, v, w, false);
         else
@@ -196,8 +217,9 @@ struct ZPass {
         chi_apply<T>((const cpx<T>*)a.chi, a.chi_kind, a.ncell, p * a.n_out + z, a.C, v, w, true);
         for (int c = 0; c < a.C; ++c) v[c] = w[c];
       }
+      const int zo = (a.perm && !a.inv) ? mirror_position(z, a.n_out / 2, a.n_out) : z;
       for (int c = 0; c < a.C; ++c) {
-        const size_t o = c * a.out_cs + p * a.out_ps + z;
+        const size_t o = c * a.out_cs + p * a.out_ps + zo;
         out[o] = (a.epi == EPI_STORE) ? v[c] : src[o] - v[c];
       }
     }
@@ -270,6 +292,7 @@ struct XArgs {
   const void* khat;
   int adj;
   int batch;
+  int perm;            // z axis of W2 in mirror-interleaved order
 };
 
 template <typename T>
@@ -335,7 +358,9 @@ struct XPass {
       const int col = e / L, kx = e - col * L;
       const long long w = w0 + col;
       if (w >= a.W) continue;
-      const int ky = (int)(w / a.L[2]), kz = (int)(w - (long long)ky * a.L[2]);
+      const int ky = (int)(w / a.L[2]);
+      int kz = (int)(w - (long long)ky * a.L[2]);
+      if (a.perm) kz = mirror_frequency(kz, a.N[2], a.L[2]);
       cpx<T> f[3];
       for (int c = 0; c < a.C; ++c) f[c] = sp[(size_t)(c * a.T + col) * a.ps + kx];
       int N[3] = {a.N[0], a.N[1], a.N[2]}, Ld[3] = {a.L[0], a.L[1], a.L[2]};

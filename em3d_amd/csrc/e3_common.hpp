@@ -17,6 +17,15 @@
 #define E3_HD inline
 #endif
 
+// Scheduling fence for the device compiler: instructions are not moved across
+// it (bounds how many loads the scheduler hoists, and so the register peak).
+// The host build of these headers (CPU tests) compiles it to nothing.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define E3_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define E3_SCHED_FENCE() ((void)0)
+#endif
+
 namespace e3 {
 
 template <typename T>

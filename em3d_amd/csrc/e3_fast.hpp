@@ -17,6 +17,8 @@
 // state) separated by block barriers, so the CPU tests can replay them.
 #pragma once
 
+#include <type_traits>
+
 #include "e3_blocks.hpp"
 
 namespace e3 {
@@ -82,6 +84,15 @@ E3_HD void fft_reg(cpx<T>* v) {
   }
 }
 
+// Element at a block-uniform base plus a per-lane byte offset below 4 GiB:
+// written this way the device compiler addresses it as scalar base + one
+// 32-bit vector offset shared by all rows, instead of a 64-bit vector
+// address per row.
+template <typename P>
+E3_HD P* at_offset(P* uniform_base, unsigned lane_bytes) {
+  return reinterpret_cast<P*>(reinterpret_cast<typename std::conditional<std::is_const<P>::value, const char, char>::type*>(uniform_base) + lane_bytes);
+}
+
 template <typename T, bool INV>
 E3_HD cpx<T> twiddle(const cpx<T>* tw, int idx, cpx<T> x) {
   const cpx<T> w = tw[idx];
@@ -143,9 +154,10 @@ struct FastS {
     const long long w = w0 + col;
     const bool ok = w < a.W;
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)q * a.in_qs;
+    const unsigned lane = (unsigned)(((long long)j * a.W + w) * sizeof(cpx<T>));
 #pragma unroll
     for (int m = 0; m < R1; ++m)
-      s.v[m] = (m < MA && ok) ? in[(size_t)(j + m * R2) * a.W + w] : czero<T>();
+      s.v[m] = (m < MA && ok) ? *at_offset(in + (size_t)m * R2 * a.W, lane) : czero<T>();
     fft_reg<T, R1, INV>(s.v);
 #pragma unroll
     for (int k = 0; k < R1; ++k) lds[(k * R2 + j) * TCP + col] = s.v[k];
@@ -166,8 +178,9 @@ struct FastS {
       }
       fft_reg<T, R2, INV>(v);
       if (w < a.W) {
+        const unsigned lane = (unsigned)(((long long)jp * a.W + w) * sizeof(cpx<T>));
 #pragma unroll
-        for (int m = 0; m < MB; ++m) out[(size_t)(jp + m * R1) * a.W + w] = v[m];
+        for (int m = 0; m < MB; ++m) *at_offset(out + (size_t)m * R1 * a.W, lane) = v[m];
       }
     }
   }
@@ -175,100 +188,143 @@ struct FastS {
 
 // ---------------------------------------------------------------------------
 // x pass: forward (Nx = L/2 live rows), octant spectral multiply, inverse,
-// Nx rows kept, in place on W2 viewed per (b, c) as (Nx, W = Ly Lz).
+// Nx rows kept, in place on W2 viewed per (b, c) as (Nx, Ly, Lz).  A block
+// takes a TC-wide column tile of the (mirror-interleaved) z axis in the two
+// rows ky and Ly - ky (rows 0 and Ny for pair 0): the four frequency
+// columns (+-ky, +-kz) of one octant entry sit in one block, so each octant
+// entry is fetched once per pass.  All components' inputs are loaded up
+// front.
 // ---------------------------------------------------------------------------
 template <typename T, int L, int R1_, int TC_, int C>
 struct FastX {
   using Real = T;
-  static constexpr int R1 = R1_, R2 = L / R1_, TC = TC_, NT = TC * R2, TCP = TC + 1;
+  static constexpr int kMulGroup = 4;   // spectral multiplies scheduled together
+  static constexpr int R1 = R1_, R2 = L / R1_, TC = TC_, TCP = TC + 1;
+  static constexpr int NTH = TC * R2;          // threads per row segment
+  static constexpr int NT = 2 * NTH;
   static constexpr int NB = R1 / R2;
-  static constexpr int LDS = L * TCP;
+  static constexpr int SEG = L * TCP;          // LDS elements per row segment
+  static constexpr int LDS = 2 * SEG + L;      // two segments + twiddle table
   struct St {
+    cpx<T> in[C][R1 / 2];
     cpx<T> X[C][R1];   // forward spectrum: X[c][b*R2 + m] at kx = (j + b R2) + m R1
   };
   const XArgs& a;
-  long long w0;
-  int b;
+  int ky0, ky1;        // the two rows of the block (scalars: no indexed member array)
+  int pos0, b;
   cpx<T>* lds;
-  E3_HD FastX(const XArgs& args, long long colblk, int bidx, cpx<T>* l)
-      : a(args), w0(colblk * TC), b(bidx), lds(l) {}
+  E3_HD FastX(const XArgs& args, long long blk, int bidx, cpx<T>* l) : a(args), b(bidx), lds(l) {
+    const int ntile = args.L[2] / TC;
+    const int pr = (int)(blk / ntile);
+    pos0 = (int)(blk - (long long)pr * ntile) * TC;
+    ky0 = pr;
+    ky1 = pr == 0 ? args.N[1] : args.L[1] - pr;
+  }
+  static long long blocks(const XArgs& a) { return (long long)a.N[1] * (a.L[2] / TC); }
 
+  struct Lane {
+    int h, col, j, ky;
+    long long w;
+  };
+  E3_HD Lane lane(int tid) const {
+    Lane q;
+    q.h = tid / NTH;
+    const int r = tid - q.h * NTH;
+    q.col = r % TC;
+    q.j = r / TC;
+    q.ky = q.h ? ky1 : ky0;
+    q.w = (long long)q.ky * a.L[2] + pos0 + q.col;
+    return q;
+  }
   E3_HD cpx<T>* plane(int c) const { return (cpx<T>*)a.data + (size_t)b * a.bs + (size_t)c * a.cs; }
+  E3_HD cpx<T>* seg(int h) const { return lds + h * SEG; }
 
-  E3_HD void fwd_a(int c, int tid) const {
-    const int col = tid % TC, j = tid / TC;
-    const long long w = w0 + col;
-    const bool ok = w < a.W;
-    const cpx<T>* d = plane(c);
+  // twiddle table exp(-2 pi i t / L) kept in LDS: re-read where used rather
+  // than held in registers across the component loop
+  E3_HD const cpx<T>* twl() const { return lds + 2 * SEG; }
+  E3_HD void load(int tid, St& s) const {
+    for (int t = tid; t < L; t += NT) lds[2 * SEG + t] = ((const cpx<T>*)a.tw)[t];
+    const Lane q = lane(tid);
+    const unsigned lane_b = (unsigned)(((long long)q.j * a.W + q.w) * sizeof(cpx<T>));
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const cpx<T>* d = plane(c);
+#pragma unroll
+      for (int m = 0; m < R1 / 2; ++m) s.in[c][m] = *at_offset(d + (size_t)m * R2 * a.W, lane_b);
+    }
+  }
+  E3_HD void fwd_a(int c, int tid, St& s) const {
+    const Lane q = lane(tid);
     cpx<T> v[R1];
 #pragma unroll
-    for (int m = 0; m < R1; ++m) v[m] = (m < R1 / 2 && ok) ? d[(size_t)(j + m * R2) * a.W + w] : czero<T>();
+    for (int m = 0; m < R1; ++m) v[m] = m < R1 / 2 ? s.in[c][m] : czero<T>();
     fft_reg<T, R1, false>(v);
+    cpx<T>* t = seg(q.h);
 #pragma unroll
-    for (int k = 0; k < R1; ++k) lds[(k * R2 + j) * TCP + col] = v[k];
+    for (int k = 0; k < R1; ++k) t[(k * R2 + q.j) * TCP + q.col] = v[k];
   }
   E3_HD void fwd_b(int c, int tid, St& s) const {
-    const int col = tid % TC, j = tid / TC;
-    const cpx<T>* tw = (const cpx<T>*)a.tw;
+    const Lane q = lane(tid);
+    const cpx<T>* tw = twl();
+    const cpx<T>* t = seg(q.h);
 #pragma unroll
     for (int bb = 0; bb < NB; ++bb) {
-      const int jp = j + bb * R2;
+      const int jp = q.j + bb * R2;
       cpx<T>* v = s.X[c] + bb * R2;
 #pragma unroll
       for (int r = 0; r < R2; ++r) {
-        const cpx<T> x = lds[(jp * R2 + r) * TCP + col];
+        const cpx<T> x = t[(jp * R2 + r) * TCP + q.col];
         v[r] = r ? twiddle<T, false>(tw, jp * r, x) : x;
       }
       fft_reg<T, R2, false>(v);
     }
   }
   E3_HD void multiply(int tid, St& s) const {
-    const int col = tid % TC, j = tid / TC;
-    const long long w = w0 + col;
-    if (w >= a.W) return;
-    const int ky = (int)(w / a.L[2]), kz = (int)(w - (long long)ky * a.L[2]);
+    const Lane q = lane(tid);
+    const int kz = a.perm ? mirror_frequency(pos0 + q.col, a.N[2], a.L[2]) : pos0 + q.col;
     int N[3] = {a.N[0], a.N[1], a.N[2]}, Ld[3] = {a.L[0], a.L[1], a.L[2]};
 #pragma unroll
     for (int bb = 0; bb < NB; ++bb) {
 #pragma unroll
       for (int m = 0; m < R2; ++m) {
-        const int kx = j + bb * R2 + m * R1;
+        if (m % kMulGroup == 0) E3_SCHED_FENCE();
+        const int kx = q.j + bb * R2 + m * R1;
         cpx<T> f[3];
 #pragma unroll
         for (int c = 0; c < C; ++c) f[c] = s.X[c][bb * R2 + m];
-        spectral_multiply<T>((const cpx<T>*)a.khat, C, N, Ld, kx, ky, kz, a.adj != 0, f);
+        spectral_multiply<T>((const cpx<T>*)a.khat, C, N, Ld, kx, q.ky, kz, a.adj != 0, f);
 #pragma unroll
         for (int c = 0; c < C; ++c) s.X[c][bb * R2 + m] = f[c];
       }
     }
   }
   E3_HD void inv_a(int c, int tid, St& s) const {
-    const int col = tid % TC, j = tid / TC;
+    const Lane q = lane(tid);
+    cpx<T>* t = seg(q.h);
 #pragma unroll
     for (int bb = 0; bb < NB; ++bb) {
-      const int jp = j + bb * R2;
+      const int jp = q.j + bb * R2;
       cpx<T>* v = s.X[c] + bb * R2;
       fft_reg<T, R2, true>(v);
 #pragma unroll
-      for (int m2 = 0; m2 < R2; ++m2) lds[(m2 * R1 + jp) * TCP + col] = v[m2];
+      for (int m2 = 0; m2 < R2; ++m2) t[(m2 * R1 + jp) * TCP + q.col] = v[m2];
     }
   }
   E3_HD void inv_b(int c, int tid) const {
-    const int col = tid % TC, j = tid / TC;
-    const long long w = w0 + col;
-    const cpx<T>* tw = (const cpx<T>*)a.tw;
+    const Lane q = lane(tid);
+    const cpx<T>* tw = twl();
+    const cpx<T>* t = seg(q.h);
     cpx<T> v[R1];
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
-      const cpx<T> x = lds[(j * R1 + m) * TCP + col];
-      v[m] = m ? twiddle<T, true>(tw, j * m, x) : x;
+      const cpx<T> x = t[(q.j * R1 + m) * TCP + q.col];
+      v[m] = m ? twiddle<T, true>(tw, q.j * m, x) : x;
     }
     fft_reg<T, R1, true>(v);
-    if (w < a.W) {
-      cpx<T>* d = plane(c);
+    cpx<T>* d = plane(c);
+    const unsigned lane_b = (unsigned)(((long long)q.j * a.W + q.w) * sizeof(cpx<T>));
 #pragma unroll
-      for (int m = 0; m < R1 / 2; ++m) d[(size_t)(j + m * R2) * a.W + w] = v[m];
-    }
+    for (int m = 0; m < R1 / 2; ++m) *at_offset(d + (size_t)m * R2 * a.W, lane_b) = v[m];
   }
 };
 
@@ -336,7 +392,7 @@ struct FastZF {
       fft_reg<T, R2, false>(v);
       if (p < a.P) {
 #pragma unroll
-        for (int m = 0; m < R2; ++m) out[jp + m * R1] = v[m];
+        for (int m = 0; m < R2; ++m) out[a.perm ? mirror_position(jp + m * R1, L / 2, L) : jp + m * R1] = v[m];
       }
     }
   }
@@ -352,7 +408,7 @@ struct FastZI {
   static constexpr int R1 = R1_, R2 = L / R1_, P = P_, NT = P * R2, R2P = R2 + 1;
   static constexpr int PS = R1 * R2P;
   static constexpr int NB = R1 / R2;
-  static constexpr int LDS = P * PS;
+  static constexpr int LDS = P * PS + L;       // pencil tiles + twiddle table
   static constexpr int MH = R2 / 2;
   struct St {
     cpx<T> y[C][NB][MH];
@@ -364,20 +420,25 @@ struct FastZI {
   E3_HD FastZI(const ZArgs& args, long long blk, int bidx, cpx<T>* l) : a(args), p0(blk * P), b(bidx), lds(l) {}
 
   E3_HD void stage_a(int c, int tid) const {
+    if (c == 0)
+      for (int t = tid; t < L; t += NT) lds[P * PS + t] = ((const cpx<T>*)a.tw)[t];
     const int j = tid % R2, pl = tid / R2;
     const long long p = p0 + pl;
     const bool ok = p < a.P;
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs + (size_t)c * a.in_cs + (size_t)p * a.in_ps;
     cpx<T> v[R1];
 #pragma unroll
-    for (int m = 0; m < R1; ++m) v[m] = ok ? in[j + m * R2] : czero<T>();
+    for (int m = 0; m < R1; ++m) {
+      const int n = j + m * R2;
+      v[m] = ok ? in[a.perm ? mirror_position(n, L / 2, L) : n] : czero<T>();
+    }
     fft_reg<T, R1, true>(v);
 #pragma unroll
     for (int k = 0; k < R1; ++k) lds[pl * PS + k * R2P + j] = v[k];
   }
   E3_HD void stage_b(int c, int tid, St& s) const {
     const int j = tid % R2, pl = tid / R2;
-    const cpx<T>* tw = (const cpx<T>*)a.tw;
+    const cpx<T>* tw = lds + P * PS;
 #pragma unroll
     for (int bb = 0; bb < NB; ++bb) {
       const int jp = j + bb * R2;
@@ -451,7 +512,7 @@ bool fast_s_len(Run& run, const SArgs& a) {
 template <typename T, int L, class Run>
 bool fast_x_len(Run& run, const XArgs& a) {
   using Cfg = FastCfg<T, L>;
-  if (a.N[0] * 2 != L) return false;
+  if (a.N[0] * 2 != L || a.L[2] % Cfg::TC != 0) return false;
   if (a.C == 3) run.template x<FastX<T, L, Cfg::R1, Cfg::TC, 3>>(a);
   else run.template x<FastX<T, L, Cfg::R1, Cfg::TC, 1>>(a);
   return true;

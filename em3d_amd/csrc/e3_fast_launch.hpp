@@ -28,9 +28,10 @@ __global__ __launch_bounds__(K::NT) void k_fast_x(XArgs a, long long ncb) {
   const int b = (int)(bid / ncb);
   K k(a, bid - (long long)b * ncb, b, lds);
   typename K::St st;
+  k.load(threadIdx.x, st);
 #pragma unroll
   for (int c = 0; c < C; ++c) {
-    k.fwd_a(c, threadIdx.x);
+    k.fwd_a(c, threadIdx.x, st);
     __syncthreads();
     k.fwd_b(c, threadIdx.x, st);
     __syncthreads();
@@ -100,7 +101,7 @@ struct HipFastRun {
   }
   template <class K>
   void x(const XArgs& a) {
-    const long long ncb = (a.W + K::TC - 1) / K::TC;
+    const long long ncb = K::blocks(a);
     hipLaunchKernelGGL((k_fast_x<K, CompOf<K>::value>), dim3((unsigned)(ncb * a.batch)), dim3(K::NT), 0, st,
                        a, ncb);
     err = hipGetLastError();

@@ -121,7 +121,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.out_ps = Lz; z.out_cs = P * Lz; z.out_bs = C * P * Lz;
   z.n_in = Nz; z.n_out = Lz; z.inv = 0;
   z.pro = adjoint ? PRO_COPY : PRO_CHI; z.epi = EPI_STORE;
-  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz;
+  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.perm = 1;
   z_pass(go, p, prec, twz, p.ax[2], z);
 
   // P2: y forward (Ny live rows -> Ly)
@@ -137,7 +137,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   std::memset(&x, 0, sizeof(x));
   x.data = W2; x.C = p.C; x.W = (long long)Ly * Lz; x.cs = (long long)Nx * x.W; x.bs = C * x.cs;
   for (int a = 0; a < 3; ++a) { x.N[a] = p.N[a]; x.L[a] = p.L[a]; }
-  x.plan = p.ax[0]; x.tw = twx; x.khat = khat; x.adj = adjoint ? 1 : 0; x.batch = p.batch;
+  x.plan = p.ax[0]; x.tw = twx; x.khat = khat; x.adj = adjoint ? 1 : 0; x.batch = p.batch; x.perm = 1;
   x.ps = Lx + 1;
   x.T = tile_count(lds_bytes(p.C, x.ps, prec), 16);
   if (x.T > x.W) x.T = (int)x.W;
@@ -155,7 +155,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.out_ps = Nz; z.out_cs = P * Nz; z.out_bs = C * P * Nz;
   z.n_in = Lz; z.n_out = Nz; z.inv = 1;
   z.pro = PRO_COPY; z.epi = adjoint ? EPI_SUB_CHIH : EPI_SUB;
-  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.u = u;
+  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.u = u; z.perm = 1;
   z_pass(go, p, prec, twz, p.ax[2], z);
 }
 

@@ -40,6 +40,8 @@ def load(path: str | None = None):
                               C.c_void_p, C.c_int]
     lib.emu_last_error.restype = C.c_char_p
     lib.emu_set_fast.argtypes = [C.c_int]
+    lib.emu_fast_calls.argtypes = [C.c_int]
+    lib.emu_fast_calls.restype = C.c_longlong
     if path is None:
         _LIB = lib
     return lib

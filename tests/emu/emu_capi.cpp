@@ -30,6 +30,8 @@ void poison(std::vector<unsigned char>& mem) {
 
 // Replays the fast (power-of-two) kernels: per-thread register state lives
 // in an array indexed by thread id; barriers separate the phase loops.
+long long g_fast_calls[4] = {0, 0, 0, 0};   // s, x, zf, zi
+
 struct EmuFastRun {
   template <class K>
   static std::vector<unsigned char> tile() {
@@ -40,6 +42,7 @@ struct EmuFastRun {
   template <class K>
   void s(const SArgs& a) {
     using T = typename K::Real;
+    ++g_fast_calls[0];
     const long long ncb = (a.W + K::TC - 1) / K::TC;
     for (long long bid = 0; bid < ncb * a.Q; ++bid) {
       auto mem = tile<K>();
@@ -53,15 +56,17 @@ struct EmuFastRun {
   template <class K>
   void x(const XArgs& a) {
     using T = typename K::Real;
-    const long long ncb = (a.W + K::TC - 1) / K::TC;
+    ++g_fast_calls[1];
+    const long long ncb = K::blocks(a);
     const int C = a.C;
     for (long long bid = 0; bid < ncb * a.batch; ++bid) {
       auto mem = tile<K>();
       const int b = (int)(bid / ncb);
       K k(a, bid - (long long)b * ncb, b, (cpx<T>*)mem.data());
       std::vector<typename K::St> st(K::NT);
+      for (int t = 0; t < K::NT; ++t) k.load(t, st[t]);
       for (int c = 0; c < C; ++c) {
-        for (int t = 0; t < K::NT; ++t) k.fwd_a(c, t);
+        for (int t = 0; t < K::NT; ++t) k.fwd_a(c, t, st[t]);
         for (int t = 0; t < K::NT; ++t) k.fwd_b(c, t, st[t]);
       }
       for (int t = 0; t < K::NT; ++t) k.multiply(t, st[t]);
@@ -74,6 +79,7 @@ struct EmuFastRun {
   template <class K>
   void zf(const ZArgs& a) {
     using T = typename K::Real;
+    ++g_fast_calls[2];
     const long long nblk = (a.P + K::P - 1) / K::P;
     for (long long bid = 0; bid < nblk * a.batch; ++bid) {
       auto mem = tile<K>();
@@ -88,6 +94,7 @@ struct EmuFastRun {
   template <class K>
   void zi(const ZArgs& a) {
     using T = typename K::Real;
+    ++g_fast_calls[3];
     const long long nblk = (a.P + K::P - 1) / K::P;
     for (long long bid = 0; bid < nblk * a.batch; ++bid) {
       auto mem = tile<K>();
@@ -189,6 +196,8 @@ extern "C" {
 const char* emu_last_error(void) { return g_err.c_str(); }
 
 void emu_set_fast(int on) { g_use_fast = on; }
+
+long long emu_fast_calls(int kind) { return g_fast_calls[kind]; }
 
 void* emu_plan_create(int nx, int ny, int nz, int ncomp, int batch, int precision) {
   EmuPlan* h = new EmuPlan();

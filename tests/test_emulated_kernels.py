@@ -111,7 +111,9 @@ print("ok")
 
 
 FAST_CASES = [((8, 8, 8), (1.0, 1.0, 1.0), 4.0), ((16, 8, 4), (1.0, 0.8, 0.6), 2.0),
-              ((8, 16, 32), (1.0, 1.0, 1.0), 3.0), ((5, 8, 16), (1.0, 1.0, 1.0), 3.0)]
+              ((8, 16, 32), (1.0, 1.0, 1.0), 3.0), ((5, 8, 16), (1.0, 1.0, 1.0), 3.0),
+              ((8, 4, 32), (1.0, 0.7, 1.3), 4.0), ((16, 6, 32), (1.0, 0.8, 0.6), 2.0),
+              ((32, 8, 64), (1.0, 1.0, 1.0), 6.0)]
 
 
 @pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("single", 1e-6)])
@@ -120,6 +122,7 @@ def test_fast_power_of_two_kernels_match_oracle(lib, N, L, k, prec, tol):
     """Power-of-two doubled axes take the register-resident two-stage kernels
     (e3_fast.hpp); mixed shapes combine them with the generic passes."""
     lib.emu_set_fast(1)
+    before = [lib.emu_fast_calls(i) for i in range(4)]
     chi = _rand((3, 3) + N, 8) * 0.3
     u = _rand((2, 3) + N, 9)
     op = emu.EmuOperator(lib, N, L, k, chi, 3, precision=prec, batch=2)
@@ -133,6 +136,11 @@ def test_fast_power_of_two_kernels_match_oracle(lib, N, L, k, prec, tol):
     opa = emu.EmuOperator(lib, N, L, k, ca, 1, precision=prec)
     assert _rel(opa.matvec(ua), vie.acoustic_matvec(ca, ua, vie.acoustic_kernel_hat(N, L, k))) < tol
     assert _rel(opa.rmatvec(ua), vie.acoustic_rmatvec(ca, ua, vie.acoustic_kernel_hat(N, L, k, adjoint=True))) < tol
+    used = [lib.emu_fast_calls(i) - before[i] for i in range(4)]
+    if 2 * N[2] >= 16:
+        assert used[2] > 0 and used[3] > 0      # z passes take the fast kernels
+    if N[2] >= 32 and N[0] <= 32:
+        assert used[1] > 0                       # paired x pass exercised
 
 
 def test_generic_and_fast_paths_agree(lib):
