@@ -295,6 +295,36 @@ struct XArgs {
   int perm;            // z axis of W2 in mirror-interleaved order
 };
 
+// ---------------------------------------------------------------------------
+// Octant spectrum layout.  Entries (comp, ix, iy, iz), 0 <= i* <= N*, are
+// stored as
+//   main  [iy][iz / S][ix][comp][iz % S]   for iz <  Nz   (groups of S = 64 B)
+//   nyq   [iy][ix][comp]                   for iz == Nz   (after main)
+// so that the entries one x-pass block needs (one iy, S consecutive iz, all
+// ix and components) are one contiguous run.  S = 8 (complex64) or 4
+// (complex128); the last iz group is zero padded when S does not divide Nz.
+// ---------------------------------------------------------------------------
+struct KhatLayout {
+  int nx1, ny1, nz, K, S, G;
+  E3_HD long long main_elems() const { return (long long)ny1 * G * nx1 * K * S; }
+  E3_HD long long total() const { return main_elems() + (long long)ny1 * nx1 * K; }
+  E3_HD long long index(int comp, int ix, int iy, int iz) const {
+    if (iz < nz) return ((((long long)iy * G + iz / S) * nx1 + ix) * K + comp) * S + iz % S;
+    return main_elems() + ((long long)iy * nx1 + ix) * K + comp;
+  }
+};
+
+E3_HD KhatLayout khat_layout(const int N[3], int ncomp, int elem_bytes) {
+  KhatLayout l;
+  l.nx1 = N[0] + 1;
+  l.ny1 = N[1] + 1;
+  l.nz = N[2];
+  l.K = ncomp == 3 ? 6 : 1;
+  l.S = 64 / elem_bytes;
+  l.G = (N[2] + l.S - 1) / l.S;
+  return l;
+}
+
 template <typename T>
 E3_HD void spectral_multiply(const cpx<T>* K, int C, int N[3], int L[3], int kx, int ky, int kz,
                              bool adj, cpx<T>* f) {
@@ -302,17 +332,16 @@ E3_HD void spectral_multiply(const cpx<T>* K, int C, int N[3], int L[3], int kx,
   const int ix = fold_index(kx, N[0], L[0], &fx);
   const int iy = fold_index(ky, N[1], L[1], &fy);
   const int iz = fold_index(kz, N[2], L[2], &fz);
-  const size_t oct = (size_t)(N[0] + 1) * (N[1] + 1) * (N[2] + 1);
-  const size_t at = ((size_t)ix * (N[1] + 1) + iy) * (N[2] + 1) + iz;
+  const KhatLayout lay = khat_layout(N, C, (int)sizeof(cpx<T>));
   if (C == 1) {
-    cpx<T> k = K[at];
+    cpx<T> k = K[lay.index(0, ix, iy, iz)];
     if (adj) k = conj(k);
     f[0] = k * f[0];
     return;
   }
   cpx<T> k[6];
   for (int i = 0; i < 6; ++i) {
-    k[i] = K[i * oct + at];
+    k[i] = K[lay.index(i, ix, iy, iz)];
     if (adj) k[i] = conj(k[i]);
   }
   const T sx = fx ? T(-1) : T(1), sy = fy ? T(-1) : T(1), sz = fz ? T(-1) : T(1);
@@ -450,6 +479,7 @@ E3_HD void kernel_point(const KBArgs& a, long long idx) {
   out[5 * D + idx] = (al[1] * al[2]) * w1;
 }
 
+// idx runs over the logical entries (comp, ix, iy, iz), iz fastest.
 template <typename T>
 E3_HD void octant_point(const KBArgs& a, long long idx) {
   const cpx<double>* full = (const cpx<double>*)a.full;
@@ -463,7 +493,8 @@ E3_HD void octant_point(const KBArgs& a, long long idx) {
   const int ix = (int)(r / (nz1 * ny1));
   const long long D = (long long)a.L[0] * a.L[1] * a.L[2];
   const cpx<double> v = full[c * D + ((long long)ix * a.L[1] + iy) * a.L[2] + iz];
-  oct[idx] = mk((T)(v.re * a.scale), (T)(v.im * a.scale));
+  const KhatLayout lay = khat_layout(a.N, a.kind == 0 ? 3 : 1, (int)sizeof(cpx<T>));
+  oct[lay.index(c, ix, iy, iz)] = mk((T)(v.re * a.scale), (T)(v.im * a.scale));
 }
 
 }  // namespace e3

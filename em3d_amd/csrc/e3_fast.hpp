@@ -188,143 +188,295 @@ struct FastS {
 
 // ---------------------------------------------------------------------------
 // x pass: forward (Nx = L/2 live rows), octant spectral multiply, inverse,
-// Nx rows kept, in place on W2 viewed per (b, c) as (Nx, Ly, Lz).  A block
-// takes a TC-wide column tile of the (mirror-interleaved) z axis in the two
-// rows ky and Ly - ky (rows 0 and Ny for pair 0): the four frequency
-// columns (+-ky, +-kz) of one octant entry sit in one block, so each octant
-// entry is fetched once per pass.  All components' inputs are loaded up
-// front.
+// Nx rows kept, in place on W2 viewed per (b, c) as (Nx, Ly, Lz).
+//
+// A block takes one octant row iy = p (0 <= p <= Ny) and one group of S
+// octant columns iz (a TC = 2S wide tile of the mirror-interleaved z axis):
+// the frequency rows ky = p and ky = Ly - p (one row for p = 0 and p = Ny),
+// one after the other, and in each the columns kz and Lz - kz.  Those four
+// frequency columns share one octant entry, so the block's part of the
+// spectrum is one contiguous run of (Nx+1) K S entries (plus the Nyquist
+// column for tile 0), copied once into LDS: every octant entry is fetched
+// once per pass.
+//
+// The transform along x is a three-stage Stockham FFT, L = RA * RB * RC with
+// RC = 8, so each thread keeps only 8 frequencies per component between the
+// forward transform, the spectral product and the inverse transform (the
+// forward last stage and the inverse first stage share one butterfly
+// mapping).  Stages exchange through LDS, in place, with a barrier between
+// the reads and the writes of a stage.
 // ---------------------------------------------------------------------------
-template <typename T, int L, int R1_, int TC_, int C>
+template <int L>
+struct XRadix;
+template <> struct XRadix<16> { static constexpr int A = 2, B = 1, C = 8; };
+template <> struct XRadix<32> { static constexpr int A = 4, B = 1, C = 8; };
+template <> struct XRadix<64> { static constexpr int A = 8, B = 1, C = 8; };
+template <> struct XRadix<128> { static constexpr int A = 4, B = 4, C = 8; };
+template <> struct XRadix<256> { static constexpr int A = 8, B = 4, C = 8; };
+template <> struct XRadix<512> { static constexpr int A = 8, B = 8, C = 8; };
+
+template <typename T, int L, int C>
 struct FastX {
   using Real = T;
-  static constexpr int kMulGroup = 4;   // spectral multiplies scheduled together
-  static constexpr int R1 = R1_, R2 = L / R1_, TC = TC_, TCP = TC + 1;
-  static constexpr int NTH = TC * R2;          // threads per row segment
-  static constexpr int NT = 2 * NTH;
-  static constexpr int NB = R1 / R2;
-  static constexpr int SEG = L * TCP;          // LDS elements per row segment
-  static constexpr int LDS = 2 * SEG + L;      // two segments + twiddle table
+  static constexpr int RA = XRadix<L>::A, RB = XRadix<L>::B, RC = XRadix<L>::C;
+  static constexpr int NTC = L / RC;                   // threads per column
+  static constexpr int S = 64 / (int)sizeof(cpx<T>);   // octant group: 64 bytes of iz
+  static constexpr int TC = 2 * S, TCP = TC + 1;
+  static constexpr int NT = TC * NTC;                  // threads of a block (one row at a time)
+  static constexpr int BA = RC / RA, BB = RB > 1 ? RC / RB : 0;   // butterflies per thread
+  static constexpr int K = C == 3 ? 6 : 1;
+  static constexpr int NX1 = L / 2 + 1;
+  static constexpr int SEG = L * TCP;                  // LDS elements of the row tile
+  static constexpr int KS_MAIN = NX1 * K * S, KS = NX1 * K * (S + 1);
+  static constexpr bool STAGE = (size_t)(SEG + L + KS) * sizeof(cpx<T>) <= 160 * 1024;
+  static constexpr int LDS = SEG + L + (STAGE ? KS : 0);
+  static constexpr int kMulGroup = 2;                  // spectral products scheduled together
+  static_assert(RA * (RB > 1 ? RB : 1) * RC == L, "radix plan");
+  static_assert(RC % RA == 0 && (RB == 1 || RC % RB == 0), "radices must divide 8");
   struct St {
-    cpx<T> in[C][R1 / 2];
-    cpx<T> X[C][R1];   // forward spectrum: X[c][b*R2 + m] at kx = (j + b R2) + m R1
+    cpx<T> in[C][BA][RA / 2];   // live inputs of stage A (rows < L/2)
+    cpx<T> X[C][RC];            // spectrum at kx = j + m (L / RC)
+    cpx<T> mid[BB > 0 ? BB : 1][RB > 1 ? RB : 1];   // middle stage values in flight
   };
   const XArgs& a;
-  int ky0, ky1;        // the two rows of the block (scalars: no indexed member array)
-  int pos0, b;
+  int pr, ky0, ky1, pos0, tile, b;
+  bool two;            // both rows ky0 = p and ky1 = Ly - p are live
   cpx<T>* lds;
   E3_HD FastX(const XArgs& args, long long blk, int bidx, cpx<T>* l) : a(args), b(bidx), lds(l) {
     const int ntile = args.L[2] / TC;
-    const int pr = (int)(blk / ntile);
-    pos0 = (int)(blk - (long long)pr * ntile) * TC;
+    pr = (int)(blk / ntile);
+    tile = (int)(blk - (long long)pr * ntile);
+    pos0 = tile * TC;
     ky0 = pr;
-    ky1 = pr == 0 ? args.N[1] : args.L[1] - pr;
+    ky1 = args.L[1] - pr;
+    two = pr > 0 && pr < args.N[1];
   }
-  static long long blocks(const XArgs& a) { return (long long)a.N[1] * (a.L[2] / TC); }
+  static long long blocks(const XArgs& a) { return (long long)(a.N[1] + 1) * (a.L[2] / TC); }
 
   struct Lane {
     int h, col, j, ky;
     long long w;
   };
-  E3_HD Lane lane(int tid) const {
+  // lane of thread tid while the block works on row h (0: ky = p, 1: ky = Ly - p)
+  E3_HD Lane lane(int tid, int h) const {
     Lane q;
-    q.h = tid / NTH;
-    const int r = tid - q.h * NTH;
-    q.col = r % TC;
-    q.j = r / TC;
-    q.ky = q.h ? ky1 : ky0;
+    q.h = h;
+    q.col = tid % TC;
+    q.j = tid / TC;
+    q.ky = h ? ky1 : ky0;
     q.w = (long long)q.ky * a.L[2] + pos0 + q.col;
     return q;
   }
   E3_HD cpx<T>* plane(int c) const { return (cpx<T>*)a.data + (size_t)b * a.bs + (size_t)c * a.cs; }
-  E3_HD cpx<T>* seg(int h) const { return lds + h * SEG; }
+  E3_HD cpx<T>* seg(int) const { return lds; }
+  E3_HD const cpx<T>* twl() const { return lds + SEG; }
+  E3_HD cpx<T>* kst() const { return lds + SEG + L; }
+  E3_HD cpx<T>& cell(cpx<T>* t, int idx, int col) const { return t[idx * TCP + col]; }
 
-  // twiddle table exp(-2 pi i t / L) kept in LDS: re-read where used rather
-  // than held in registers across the component loop
-  E3_HD const cpx<T>* twl() const { return lds + 2 * SEG; }
-  E3_HD void load(int tid, St& s) const {
-    for (int t = tid; t < L; t += NT) lds[2 * SEG + t] = ((const cpx<T>*)a.tw)[t];
-    const Lane q = lane(tid);
-    const unsigned lane_b = (unsigned)(((long long)q.j * a.W + q.w) * sizeof(cpx<T>));
+  // ---- loads: twiddles, the block's octant run, all components' inputs ----
+  E3_HD void stage_tables(int tid) const {
+    for (int t = tid; t < L; t += NT) lds[SEG + t] = ((const cpx<T>*)a.tw)[t];
+    if (STAGE) {
+      const int Nn[3] = {a.N[0], a.N[1], a.N[2]};
+      const KhatLayout lay = khat_layout(Nn, C, (int)sizeof(cpx<T>));
+      const cpx<T>* kg = (const cpx<T>*)a.khat + (((long long)pr * lay.G + tile) * NX1) * K * S;
+      for (int t = tid; t < KS_MAIN; t += NT) kst()[t] = kg[t];
+      if (tile == 0) {
+        const cpx<T>* kn = (const cpx<T>*)a.khat + lay.main_elems() + (long long)pr * NX1 * K;
+        for (int t = tid; t < NX1 * K; t += NT) kst()[KS_MAIN + t] = kn[t];
+      }
+    }
+  }
+  E3_HD void load(int tid, int h, St& s) const {
+    const Lane q = lane(tid, h);
+    // stage-A butterfly jj = q.j + bb NTC reads rows jj + m L/RA, live for m < RA/2
 #pragma unroll
     for (int c = 0; c < C; ++c) {
       const cpx<T>* d = plane(c);
 #pragma unroll
-      for (int m = 0; m < R1 / 2; ++m) s.in[c][m] = *at_offset(d + (size_t)m * R2 * a.W, lane_b);
-    }
-  }
-  E3_HD void fwd_a(int c, int tid, St& s) const {
-    const Lane q = lane(tid);
-    cpx<T> v[R1];
+      for (int bb = 0; bb < BA; ++bb) {
+        const int jj = q.j + bb * NTC;
+        const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
 #pragma unroll
-    for (int m = 0; m < R1; ++m) v[m] = m < R1 / 2 ? s.in[c][m] : czero<T>();
-    fft_reg<T, R1, false>(v);
-    cpx<T>* t = seg(q.h);
-#pragma unroll
-    for (int k = 0; k < R1; ++k) t[(k * R2 + q.j) * TCP + q.col] = v[k];
-  }
-  E3_HD void fwd_b(int c, int tid, St& s) const {
-    const Lane q = lane(tid);
-    const cpx<T>* tw = twl();
-    const cpx<T>* t = seg(q.h);
-#pragma unroll
-    for (int bb = 0; bb < NB; ++bb) {
-      const int jp = q.j + bb * R2;
-      cpx<T>* v = s.X[c] + bb * R2;
-#pragma unroll
-      for (int r = 0; r < R2; ++r) {
-        const cpx<T> x = t[(jp * R2 + r) * TCP + q.col];
-        v[r] = r ? twiddle<T, false>(tw, jp * r, x) : x;
-      }
-      fft_reg<T, R2, false>(v);
-    }
-  }
-  E3_HD void multiply(int tid, St& s) const {
-    const Lane q = lane(tid);
-    const int kz = a.perm ? mirror_frequency(pos0 + q.col, a.N[2], a.L[2]) : pos0 + q.col;
-    int N[3] = {a.N[0], a.N[1], a.N[2]}, Ld[3] = {a.L[0], a.L[1], a.L[2]};
-#pragma unroll
-    for (int bb = 0; bb < NB; ++bb) {
-#pragma unroll
-      for (int m = 0; m < R2; ++m) {
-        if (m % kMulGroup == 0) E3_SCHED_FENCE();
-        const int kx = q.j + bb * R2 + m * R1;
-        cpx<T> f[3];
-#pragma unroll
-        for (int c = 0; c < C; ++c) f[c] = s.X[c][bb * R2 + m];
-        spectral_multiply<T>((const cpx<T>*)a.khat, C, N, Ld, kx, q.ky, kz, a.adj != 0, f);
-#pragma unroll
-        for (int c = 0; c < C; ++c) s.X[c][bb * R2 + m] = f[c];
+        for (int m = 0; m < RA / 2; ++m)
+          s.in[c][bb][m] = *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b);
       }
     }
   }
-  E3_HD void inv_a(int c, int tid, St& s) const {
-    const Lane q = lane(tid);
+  // ---- forward stage A (Ns = 1): regs -> LDS ----
+  E3_HD void fwd_a(int c, int tid, int h, St& s) const {
+    const Lane q = lane(tid, h);
     cpx<T>* t = seg(q.h);
 #pragma unroll
-    for (int bb = 0; bb < NB; ++bb) {
-      const int jp = q.j + bb * R2;
-      cpx<T>* v = s.X[c] + bb * R2;
-      fft_reg<T, R2, true>(v);
+    for (int bb = 0; bb < BA; ++bb) {
+      const int jj = q.j + bb * NTC;
+      cpx<T> v[RA];
 #pragma unroll
-      for (int m2 = 0; m2 < R2; ++m2) t[(m2 * R1 + jp) * TCP + q.col] = v[m2];
+      for (int m = 0; m < RA; ++m) v[m] = m < RA / 2 ? s.in[c][bb][m] : czero<T>();
+      fft_reg<T, RA, false>(v);
+#pragma unroll
+      for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
     }
   }
-  E3_HD void inv_b(int c, int tid) const {
-    const Lane q = lane(tid);
-    const cpx<T>* tw = twl();
-    const cpx<T>* t = seg(q.h);
-    cpx<T> v[R1];
+  // ---- middle stage (radix RB): read | barrier | write, in place ----
+  E3_HD void mid_read(int tid, int h, St& s) const {
+    if constexpr (RB > 1) {
+      const Lane q = lane(tid, h);
+      const cpx<T>* t = seg(q.h);
 #pragma unroll
-    for (int m = 0; m < R1; ++m) {
-      const cpx<T> x = t[(q.j * R1 + m) * TCP + q.col];
-      v[m] = m ? twiddle<T, true>(tw, q.j * m, x) : x;
+      for (int bb = 0; bb < BB; ++bb) {
+        const int jj = q.j + bb * NTC;
+#pragma unroll
+        for (int m = 0; m < RB; ++m) s.mid[bb][m] = t[(jj + m * (L / RB)) * TCP + q.col];
+      }
     }
-    fft_reg<T, R1, true>(v);
+  }
+  template <bool INV>
+  E3_HD void mid_write(int tid, int h, St& s, int ns) const {
+    if constexpr (RB > 1) {
+      const Lane q = lane(tid, h);
+      cpx<T>* t = seg(q.h);
+      const cpx<T>* tw = twl();
+#pragma unroll
+      for (int bb = 0; bb < BB; ++bb) {
+        const int jj = q.j + bb * NTC;
+        const int k = jj % ns;
+        cpx<T>* v = s.mid[bb];
+#pragma unroll
+        for (int m = 1; m < RB; ++m) v[m] = twiddle<T, INV>(tw, k * m * (L / (ns * RB)), v[m]);
+        fft_reg<T, RB, INV>(v);
+#pragma unroll
+        for (int m = 0; m < RB; ++m) t[((jj / ns) * ns * RB + k + m * ns) * TCP + q.col] = v[m];
+      }
+    }
+  }
+  // ---- phase schedule: NPH phases separated by block barriers ----
+  // row h = 0 then row h = 1 (skipped, uniformly, when the block has one row);
+  // per row: forward per component, spectral product, inverse per component
+  static constexpr int MID = RB > 1 ? 1 : 0;
+  static constexpr int PPC = 2 + 2 * MID;              // phases per component and direction
+  static constexpr int NPR = 2 * C * PPC;              // phases per row
+  static constexpr int NPH = 2 * NPR;
+  template <int I>
+  E3_HD void phase(int tid, St& s) const {
+    constexpr int h = I / NPR;
+    constexpr int i1 = I - h * NPR;
+    constexpr bool fwd = i1 < C * PPC;
+    constexpr int ii = fwd ? i1 : i1 - C * PPC;
+    constexpr int c = ii / PPC, r = ii - c * PPC;
+    if (h == 1 && !two) return;
+    if constexpr (fwd) {
+      if constexpr (r == 0) {
+        if constexpr (c == 0) {
+          if constexpr (h == 0) stage_tables(tid);
+          load(tid, h, s);
+        }
+        fwd_a(c, tid, h, s);
+      } else if constexpr (MID && r == 1) {
+        mid_read(tid, h, s);
+      } else if constexpr (MID && r == 2) {
+        mid_write<false>(tid, h, s, RA);
+      } else {
+        fwd_c(c, tid, h, s);
+      }
+    } else {
+      if constexpr (r == 0) {
+        if constexpr (c == 0) multiply(tid, h, s);
+        inv_c(c, tid, h, s);
+      } else if constexpr (MID && r == 1) {
+        mid_read(tid, h, s);
+      } else if constexpr (MID && r == 2) {
+        mid_write<true>(tid, h, s, RC);
+      } else {
+        inv_a(c, tid, h);
+      }
+    }
+  }
+  // ---- forward last stage (radix RC, Ns = L/RC): LDS -> X regs ----
+  E3_HD void fwd_c(int c, int tid, int h, St& s) const {
+    const Lane q = lane(tid, h);
+    const cpx<T>* t = seg(q.h);
+    const cpx<T>* tw = twl();
+    cpx<T>* v = s.X[c];
+#pragma unroll
+    for (int m = 0; m < RC; ++m) {
+      const cpx<T> x = cell((cpx<T>*)t, q.j + m * NTC, q.col);
+      v[m] = m ? twiddle<T, false>(tw, q.j * m, x) : x;
+    }
+    fft_reg<T, RC, false>(v);
+  }
+  // ---- spectral product at kx = j + m NTC ----
+  E3_HD cpx<T> kentry(int comp, int ix, int iz, bool nyq, int slot) const {
+    if (STAGE) return nyq ? kst()[KS_MAIN + ix * K + comp] : kst()[(ix * K + comp) * S + slot];
+    const int Nn[3] = {a.N[0], a.N[1], a.N[2]};
+    const KhatLayout lay = khat_layout(Nn, C, (int)sizeof(cpx<T>));
+    return ((const cpx<T>*)a.khat)[lay.index(comp, ix, pr, iz)];
+  }
+  E3_HD void multiply(int tid, int h, St& s) const {
+    const Lane q = lane(tid, h);
+    const int pos = pos0 + q.col;
+    const bool nyq = pos == 1;                          // kz = Nz
+    const int iz = nyq ? a.N[2] : (pos >> 1);
+    const int slot = q.col >> 1;
+    const T sy = q.h ? T(-1) : T(1);                    // row Ly - p is the mirrored one
+    const T sz = ((pos & 1) && pos > 1) ? T(-1) : T(1); // odd positions >= 3 hold kz > Nz
+    const bool adj = a.adj != 0;
+#pragma unroll
+    for (int m = 0; m < RC; ++m) {
+      if (m % kMulGroup == 0) E3_SCHED_FENCE();        // bound the reads in flight
+      const int kx = q.j + m * NTC;
+      const bool fx = kx > L / 2;
+      const int ix = fx ? L - kx : kx;
+      if (C == 1) {
+        cpx<T> k = kentry(0, ix, iz, nyq, slot);
+        if (adj) k = conj(k);
+        s.X[0][m] = k * s.X[0][m];
+      } else {
+        cpx<T> k[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          k[i] = kentry(i, ix, iz, nyq, slot);
+          if (adj) k[i] = conj(k[i]);
+        }
+        const T sx = fx ? T(-1) : T(1);
+        const cpx<T> kxy = (sx * sy) * k[3], kxz = (sx * sz) * k[4], kyz = (sy * sz) * k[5];
+        const int c1 = C > 1 ? 1 : 0, c2 = C > 2 ? 2 : 0;
+        const cpx<T> f0 = s.X[0][m], f1 = s.X[c1][m], f2 = s.X[c2][m];
+        s.X[0][m] = k[0] * f0 + kxy * f1 + kxz * f2;
+        s.X[c1][m] = kxy * f0 + k[1] * f1 + kyz * f2;
+        s.X[c2][m] = kxz * f0 + kyz * f1 + k[2] * f2;
+      }
+    }
+  }
+  // ---- inverse first stage (radix RC, Ns = 1) from X regs -> LDS ----
+  E3_HD void inv_c(int c, int tid, int h, St& s) const {
+    const Lane q = lane(tid, h);
+    cpx<T>* t = seg(q.h);
+    cpx<T>* v = s.X[c];
+    fft_reg<T, RC, true>(v);
+#pragma unroll
+    for (int m = 0; m < RC; ++m) cell(t, q.j * RC + m, q.col) = v[m];
+  }
+  // ---- inverse last stage (radix RA, Ns = L/RA): LDS -> rows < Nx ----
+  E3_HD void inv_a(int c, int tid, int h) const {
+    const Lane q = lane(tid, h);
+    const cpx<T>* t = seg(q.h);
+    const cpx<T>* tw = twl();
     cpx<T>* d = plane(c);
-    const unsigned lane_b = (unsigned)(((long long)q.j * a.W + q.w) * sizeof(cpx<T>));
 #pragma unroll
-    for (int m = 0; m < R1 / 2; ++m) *at_offset(d + (size_t)m * R2 * a.W, lane_b) = v[m];
+    for (int bb = 0; bb < BA; ++bb) {
+      const int jj = q.j + bb * NTC;
+      cpx<T> v[RA];
+#pragma unroll
+      for (int m = 0; m < RA; ++m) {
+        const cpx<T> x = cell((cpx<T>*)t, jj + m * (L / RA), q.col);
+        v[m] = m ? twiddle<T, true>(tw, jj * m, x) : x;
+      }
+      fft_reg<T, RA, true>(v);
+      const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
+#pragma unroll
+      for (int m = 0; m < RA / 2; ++m) *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) = v[m];
+    }
   }
 };
 
@@ -511,10 +663,16 @@ bool fast_s_len(Run& run, const SArgs& a) {
 
 template <typename T, int L, class Run>
 bool fast_x_len(Run& run, const XArgs& a) {
-  using Cfg = FastCfg<T, L>;
-  if (a.N[0] * 2 != L || a.L[2] % Cfg::TC != 0) return false;
-  if (a.C == 3) run.template x<FastX<T, L, Cfg::R1, Cfg::TC, 3>>(a);
-  else run.template x<FastX<T, L, Cfg::R1, Cfg::TC, 1>>(a);
+  using X3 = FastX<T, L, 3>;
+  using X1 = FastX<T, L, 1>;
+  if (a.N[0] * 2 != L || a.L[2] % X3::TC != 0 || !a.perm) return false;
+  if (a.C == 3) {
+    if (X3::NT > 1024) return false;
+    run.template x<X3>(a);
+  } else {
+    if (X1::NT > 1024) return false;
+    run.template x<X1>(a);
+  }
   return true;
 }
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "e3_fast.hpp"
 
 namespace e3 {
@@ -20,6 +22,12 @@ __global__ __launch_bounds__(K::NT) void k_fast_s(SArgs a, long long ncb) {
   k.stage_b(threadIdx.x, st);
 }
 
+// Phases 0 .. NPH-1 of a block, each a compile-time instance, barriers between.
+template <class K, int... I>
+__device__ __forceinline__ void run_phases(K& k, typename K::St& st, std::integer_sequence<int, I...>) {
+  ((k.template phase<I>(threadIdx.x, st), (I + 1 < K::NPH ? __syncthreads() : (void)0)), ...);
+}
+
 template <class K, int C>
 __global__ __launch_bounds__(K::NT) void k_fast_x(XArgs a, long long ncb) {
   using T = typename K::Real;
@@ -28,22 +36,7 @@ __global__ __launch_bounds__(K::NT) void k_fast_x(XArgs a, long long ncb) {
   const int b = (int)(bid / ncb);
   K k(a, bid - (long long)b * ncb, b, lds);
   typename K::St st;
-  k.load(threadIdx.x, st);
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    k.fwd_a(c, threadIdx.x, st);
-    __syncthreads();
-    k.fwd_b(c, threadIdx.x, st);
-    __syncthreads();
-  }
-  k.multiply(threadIdx.x, st);
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    k.inv_a(c, threadIdx.x, st);
-    __syncthreads();
-    k.inv_b(c, threadIdx.x);
-    if (c + 1 < C) __syncthreads();
-  }
+  run_phases(k, st, std::make_integer_sequence<int, K::NPH>{});
 }
 
 template <class K>
@@ -81,8 +74,8 @@ __global__ __launch_bounds__(K::NT) void k_fast_zi(ZArgs a, long long nblk) {
 
 template <class X>
 struct CompOf;
-template <typename T, int L, int R1, int TC, int C>
-struct CompOf<FastX<T, L, R1, TC, C>> {
+template <typename T, int L, int C>
+struct CompOf<FastX<T, L, C>> {
   static constexpr int value = C;
 };
 template <typename T, int L, int R1, int P, int C>

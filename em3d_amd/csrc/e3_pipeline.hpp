@@ -27,7 +27,8 @@ struct Plan {
   const void* tw_f[3];   // forward twiddles exp(-2 pi i t / L), complex<float>
   const void* tw_d[3];   // same in complex<double>
   long long w1_elems, w2_elems;   // workspace in plan-precision complex elements
-  long long khat_elems;           // octant spectrum elements
+  long long khat_elems;           // octant spectrum storage elements (with group padding)
+  long long khat_entries;         // logical octant entries K (Nx+1)(Ny+1)(Nz+1)
   long long build_elems;          // complex<double> scratch of the kernel build
 };
 
@@ -65,7 +66,8 @@ inline std::string plan_init(Plan* p, int nx, int ny, int nz, int ncomp, int bat
   p->w1_elems = B * C * n[0] * n[1] * (long long)p->L[2];
   p->w2_elems = B * C * n[0] * (long long)p->L[1] * p->L[2];
   const long long K = ncomp == 3 ? 6 : 1;
-  p->khat_elems = K * (long long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+  p->khat_elems = khat_layout(p->N, ncomp, (int)elem_bytes(prec)).total();
+  p->khat_entries = K * (long long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
   p->build_elems = K * (long long)p->L[0] * p->L[1] * p->L[2];
   return "";
 }
@@ -194,7 +196,7 @@ void run_kernel_build(Launcher& go, const Plan& p, double k0, const double h[3],
   s.n_in = s.n_out = p.L[0];
   s_pass(go, PREC_DOUBLE, p.tw_d[0], p.ax[0], s);
 
-  go.koct(kb, p.khat_elems, p.prec);
+  go.koct(kb, p.khat_entries, p.prec);
 }
 
 }  // namespace e3

@@ -79,15 +79,40 @@ class HipPlan:
             self.handle = None
 
 
+def khat_storage_index(N, ncomp: int, elem_bytes: int):
+    """Storage index of every logical octant entry (comp, ix, iy, iz) as an int64
+    array of shape (K, Nx+1, Ny+1, Nz+1).  The library stores the octant as
+    main[iy][iz // S][ix][comp][iz % S] for iz < Nz (S = 64 bytes of entries)
+    followed by nyq[iy][ix][comp] for iz = Nz."""
+    K = 6 if ncomp == 3 else 1
+    S = 64 // elem_bytes
+    nx1, ny1, nz = N[0] + 1, N[1] + 1, N[2]
+    G = (nz + S - 1) // S
+    c = np.arange(K)[:, None, None, None]
+    ix = np.arange(nx1)[None, :, None, None]
+    iy = np.arange(ny1)[None, None, :, None]
+    iz = np.arange(nz + 1)[None, None, None, :]
+    main = ((((iy * G + iz // S) * nx1 + ix) * K + c) * S + iz % S)
+    main_elems = ny1 * G * nx1 * K * S
+    nyq = main_elems + (iy * nx1 + ix) * K + c
+    return np.where(iz < nz, main, nyq).astype(np.int64)
+
+
+def octant_entries(khat_flat, N, ncomp: int):
+    """Logical (K, Nx+1, Ny+1, Nz+1) view (a gathered copy) of a stored octant spectrum."""
+    import torch
+    idx = khat_storage_index(N, ncomp, khat_flat.element_size())
+    return khat_flat[torch.as_tensor(idx, device=khat_flat.device)]
+
+
 def build_octant_spectrum(N, L, k0: float, ncomp: int, precision: Precision, device_index: int = 0):
-    """Device octant spectrum ``(K, Nx+1, Ny+1, Nz+1)``, FP64 build, plan precision."""
+    """Device octant spectrum in the library's storage order (1-D), FP64 build,
+    plan precision, pre-scaled by 1/(8 Nx Ny Nz)."""
     import torch
     plan = HipPlan.get(N, ncomp, 1, precision, device_index)
     dev = torch.device("cuda", device_index)
     es = 8 if precision is Precision.SINGLE else 16
-    K = 6 if ncomp == 3 else 1
-    khat = torch.empty((K, N[0] + 1, N[1] + 1, N[2] + 1), dtype=precision.torch_complex_dtype, device=dev)
-    assert khat.numel() * es == plan.khat_bytes
+    khat = torch.zeros(plan.khat_bytes // es, dtype=precision.torch_complex_dtype, device=dev)
     scratch = torch.empty(plan.build_bytes, dtype=torch.uint8, device=dev)
     h = [L[a] / N[a] for a in range(3)]
     _lib.check(_lib.lib().em3d_kernel_build(plan.handle, float(k0), h[0], h[1], h[2], khat.data_ptr(),
@@ -104,9 +129,10 @@ def _fold(n: int, device):
     return torch.where(flipped, Lf - k, k), flipped
 
 
-def materialize_kernel_hat(oct_khat, N, adjoint: bool = False):
+def materialize_kernel_hat(khat_flat, N, adjoint: bool = False):
     """Full ``(3, 3, 2Nx, 2Ny, 2Nz)`` spectrum (unscaled, FFT index order) from the octant."""
     import torch
+    oct_khat = octant_entries(khat_flat, N, 3)
     dev = oct_khat.device
     (ix, fx), (iy, fy), (iz, fz) = (_fold(N[a], dev) for a in range(3))
     D = 8 * prod(N)

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../em3d_amd/csrc/e3_fast.hpp"
@@ -58,23 +59,20 @@ struct EmuFastRun {
     using T = typename K::Real;
     ++g_fast_calls[1];
     const long long ncb = K::blocks(a);
-    const int C = a.C;
     for (long long bid = 0; bid < ncb * a.batch; ++bid) {
       auto mem = tile<K>();
       const int b = (int)(bid / ncb);
       K k(a, bid - (long long)b * ncb, b, (cpx<T>*)mem.data());
       std::vector<typename K::St> st(K::NT);
-      for (int t = 0; t < K::NT; ++t) k.load(t, st[t]);
-      for (int c = 0; c < C; ++c) {
-        for (int t = 0; t < K::NT; ++t) k.fwd_a(c, t, st[t]);
-        for (int t = 0; t < K::NT; ++t) k.fwd_b(c, t, st[t]);
-      }
-      for (int t = 0; t < K::NT; ++t) k.multiply(t, st[t]);
-      for (int c = 0; c < C; ++c) {
-        for (int t = 0; t < K::NT; ++t) k.inv_a(c, t, st[t]);
-        for (int t = 0; t < K::NT; ++t) k.inv_b(c, t);
-      }
+      phases(k, st, std::make_integer_sequence<int, K::NPH>{});
     }
+  }
+  template <class K, int... I>
+  static void phases(K& k, std::vector<typename K::St>& st, std::integer_sequence<int, I...>) {
+    auto one = [&](auto ic) {
+      for (int t = 0; t < K::NT; ++t) k.template phase<decltype(ic)::value>(t, st[t]);
+    };
+    (one(std::integral_constant<int, I>{}), ...);
   }
   template <class K>
   void zf(const ZArgs& a) {
