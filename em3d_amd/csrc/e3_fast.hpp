@@ -212,29 +212,38 @@ template <> struct XRadix<16> { static constexpr int A = 2, B = 1, C = 8; };
 template <> struct XRadix<32> { static constexpr int A = 4, B = 1, C = 8; };
 template <> struct XRadix<64> { static constexpr int A = 8, B = 1, C = 8; };
 template <> struct XRadix<128> { static constexpr int A = 4, B = 4, C = 8; };
-template <> struct XRadix<256> { static constexpr int A = 8, B = 4, C = 8; };
+#ifndef E3_XR256
+#define E3_XR256 8, 4, 8
+#endif
+template <> struct XRadix<256> {
+  static constexpr int r[3] = {E3_XR256};
+  static constexpr int A = r[0], B = r[1], C = r[2];
+};
 template <> struct XRadix<512> { static constexpr int A = 8, B = 8, C = 8; };
 
-template <typename T, int L, int C>
+template <typename T, int L, int C, int TC_ = 128 / (int)sizeof(cpx<T>), bool STAGE_ = true, int WAVES_ = 2>
 struct FastX {
   using Real = T;
+  static constexpr int NCOMP = C;
   static constexpr int RA = XRadix<L>::A, RB = XRadix<L>::B, RC = XRadix<L>::C;
   static constexpr int NTC = L / RC;                   // threads per column
   static constexpr int S = 64 / (int)sizeof(cpx<T>);   // octant group: 64 bytes of iz
-  static constexpr int TC = 2 * S, TCP = TC + 1;
+  static constexpr int TC = TC_, TCP = TC + 1;        // columns per block, TC divides 2S
   static constexpr int NT = TC * NTC;                  // threads of a block (one row at a time)
   static constexpr int BA = RC / RA, BB = RB > 1 ? RC / RB : 0;   // butterflies per thread
   static constexpr int K = C == 3 ? 6 : 1;
   static constexpr int NX1 = L / 2 + 1;
   static constexpr int SEG = L * TCP;                  // LDS elements of the row tile
+  // the block's octant run: staged once in LDS (STAGE_) or read through the caches
+  static constexpr int KGRP = TC / 2;                  // octant columns of the block
   static constexpr int KS_MAIN = NX1 * K * S, KS = NX1 * K * (S + 1);
-  static constexpr bool STAGE = (size_t)(SEG + L + KS) * sizeof(cpx<T>) <= 160 * 1024;
+  static constexpr bool STAGE = STAGE_ && (size_t)(L * (TC + 1) + L + KS) * sizeof(cpx<T>) <= 160 * 1024;
+  static constexpr int MIN_WAVES_PER_EU = WAVES_;
   static constexpr int LDS = SEG + L + (STAGE ? KS : 0);
   static constexpr int kMulGroup = 2;                  // spectral products scheduled together
   static_assert(RA * (RB > 1 ? RB : 1) * RC == L, "radix plan");
-  static_assert(RC % RA == 0 && (RB == 1 || RC % RB == 0), "radices must divide 8");
+  static_assert(RC % RA == 0 && (RB == 1 || RC % RB == 0), "stage radices must divide the last one");
   struct St {
-    cpx<T> in[C][BA][RA / 2];   // live inputs of stage A (rows < L/2)
     cpx<T> X[C][RC];            // spectrum at kx = j + m (L / RC)
     cpx<T> mid[BB > 0 ? BB : 1][RB > 1 ? RB : 1];   // middle stage values in flight
   };
@@ -279,40 +288,28 @@ struct FastX {
     if (STAGE) {
       const int Nn[3] = {a.N[0], a.N[1], a.N[2]};
       const KhatLayout lay = khat_layout(Nn, C, (int)sizeof(cpx<T>));
-      const cpx<T>* kg = (const cpx<T>*)a.khat + (((long long)pr * lay.G + tile) * NX1) * K * S;
+      const cpx<T>* kg = (const cpx<T>*)a.khat + (((long long)pr * lay.G + (pos0 >> 1) / S) * NX1) * K * S;
       for (int t = tid; t < KS_MAIN; t += NT) kst()[t] = kg[t];
-      if (tile == 0) {
+      if (pos0 == 0) {
         const cpx<T>* kn = (const cpx<T>*)a.khat + lay.main_elems() + (long long)pr * NX1 * K;
         for (int t = tid; t < NX1 * K; t += NT) kst()[KS_MAIN + t] = kn[t];
       }
     }
   }
-  E3_HD void load(int tid, int h, St& s) const {
-    const Lane q = lane(tid, h);
-    // stage-A butterfly jj = q.j + bb NTC reads rows jj + m L/RA, live for m < RA/2
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      const cpx<T>* d = plane(c);
-#pragma unroll
-      for (int bb = 0; bb < BA; ++bb) {
-        const int jj = q.j + bb * NTC;
-        const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
-#pragma unroll
-        for (int m = 0; m < RA / 2; ++m)
-          s.in[c][bb][m] = *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b);
-      }
-    }
-  }
-  // ---- forward stage A (Ns = 1): regs -> LDS ----
+  // ---- forward stage A (Ns = 1): global rows < L/2 -> regs -> LDS ----
+  // stage-A butterfly jj = q.j + bb NTC reads rows jj + m L/RA, live for m < RA/2
   E3_HD void fwd_a(int c, int tid, int h, St& s) const {
     const Lane q = lane(tid, h);
     cpx<T>* t = seg(q.h);
+    const cpx<T>* d = plane(c);
 #pragma unroll
     for (int bb = 0; bb < BA; ++bb) {
       const int jj = q.j + bb * NTC;
+      const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
       cpx<T> v[RA];
 #pragma unroll
-      for (int m = 0; m < RA; ++m) v[m] = m < RA / 2 ? s.in[c][bb][m] : czero<T>();
+      for (int m = 0; m < RA; ++m)
+        v[m] = m < RA / 2 ? *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) : czero<T>();
       fft_reg<T, RA, false>(v);
 #pragma unroll
       for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
@@ -351,25 +348,23 @@ struct FastX {
     }
   }
   // ---- phase schedule: NPH phases separated by block barriers ----
-  // row h = 0 then row h = 1 (skipped, uniformly, when the block has one row);
-  // per row: forward per component, spectral product, inverse per component
+  // per row: forward per component, spectral product, inverse per component.
+  // Rows run in a runtime loop (rows() of them): the phases of a row are
+  // compile-time instances, the row index is not, so nothing is carried from
+  // one row to the next in registers.
   static constexpr int MID = RB > 1 ? 1 : 0;
   static constexpr int PPC = 2 + 2 * MID;              // phases per component and direction
-  static constexpr int NPR = 2 * C * PPC;              // phases per row
-  static constexpr int NPH = 2 * NPR;
+  static constexpr int NPH = 2 * C * PPC;              // phases per row
+  E3_HD int rows() const { return two ? 2 : 1; }
   template <int I>
-  E3_HD void phase(int tid, St& s) const {
-    constexpr int h = I / NPR;
-    constexpr int i1 = I - h * NPR;
-    constexpr bool fwd = i1 < C * PPC;
-    constexpr int ii = fwd ? i1 : i1 - C * PPC;
+  E3_HD void phase(int tid, int h, St& s) const {
+    constexpr bool fwd = I < C * PPC;
+    constexpr int ii = fwd ? I : I - C * PPC;
     constexpr int c = ii / PPC, r = ii - c * PPC;
-    if (h == 1 && !two) return;
     if constexpr (fwd) {
       if constexpr (r == 0) {
         if constexpr (c == 0) {
-          if constexpr (h == 0) stage_tables(tid);
-          load(tid, h, s);
+          if (h == 0) stage_tables(tid);
         }
         fwd_a(c, tid, h, s);
       } else if constexpr (MID && r == 1) {
@@ -406,21 +401,32 @@ struct FastX {
     fft_reg<T, RC, false>(v);
   }
   // ---- spectral product at kx = j + m NTC ----
-  E3_HD cpx<T> kentry(int comp, int ix, int iz, bool nyq, int slot) const {
-    if (STAGE) return nyq ? kst()[KS_MAIN + ix * K + comp] : kst()[(ix * K + comp) * S + slot];
+  // block-uniform bases of this block's octant run (main group, Nyquist column)
+  E3_HD const cpx<T>* krun() const {
     const int Nn[3] = {a.N[0], a.N[1], a.N[2]};
     const KhatLayout lay = khat_layout(Nn, C, (int)sizeof(cpx<T>));
-    return ((const cpx<T>*)a.khat)[lay.index(comp, ix, pr, iz)];
+    return (const cpx<T>*)a.khat + (((long long)pr * lay.G + (pos0 >> 1) / S) * NX1) * K * S;
+  }
+  E3_HD const cpx<T>* knyq() const {
+    const int Nn[3] = {a.N[0], a.N[1], a.N[2]};
+    const KhatLayout lay = khat_layout(Nn, C, (int)sizeof(cpx<T>));
+    return (const cpx<T>*)a.khat + lay.main_elems() + (long long)pr * NX1 * K;
+  }
+  E3_HD cpx<T> kentry(const cpx<T>* run, const cpx<T>* nq, int comp, int ix, bool nyq, int slot) const {
+    if (STAGE) return nyq ? kst()[KS_MAIN + ix * K + comp] : kst()[(ix * K + comp) * S + slot];
+    return nyq ? *at_offset(nq, (unsigned)((ix * K + comp) * sizeof(cpx<T>)))
+               : *at_offset(run, (unsigned)(((ix * K + comp) * S + slot) * sizeof(cpx<T>)));
   }
   E3_HD void multiply(int tid, int h, St& s) const {
     const Lane q = lane(tid, h);
     const int pos = pos0 + q.col;
     const bool nyq = pos == 1;                          // kz = Nz
-    const int iz = nyq ? a.N[2] : (pos >> 1);
-    const int slot = q.col >> 1;
+    const int slot = (pos >> 1) % S;
     const T sy = q.h ? T(-1) : T(1);                    // row Ly - p is the mirrored one
     const T sz = ((pos & 1) && pos > 1) ? T(-1) : T(1); // odd positions >= 3 hold kz > Nz
     const bool adj = a.adj != 0;
+    const cpx<T>* run = krun();
+    const cpx<T>* nq = knyq();
 #pragma unroll
     for (int m = 0; m < RC; ++m) {
       if (m % kMulGroup == 0) E3_SCHED_FENCE();        // bound the reads in flight
@@ -428,14 +434,14 @@ struct FastX {
       const bool fx = kx > L / 2;
       const int ix = fx ? L - kx : kx;
       if (C == 1) {
-        cpx<T> k = kentry(0, ix, iz, nyq, slot);
+        cpx<T> k = kentry(run, nq, 0, ix, nyq, slot);
         if (adj) k = conj(k);
         s.X[0][m] = k * s.X[0][m];
       } else {
         cpx<T> k[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-          k[i] = kentry(i, ix, iz, nyq, slot);
+          k[i] = kentry(run, nq, i, ix, nyq, slot);
           if (adj) k[i] = conj(k[i]);
         }
         const T sx = fx ? T(-1) : T(1);
@@ -557,6 +563,7 @@ struct FastZF {
 template <typename T, int L, int R1_, int P_, int C>
 struct FastZI {
   using Real = T;
+  static constexpr int NCOMP = C;
   static constexpr int R1 = R1_, R2 = L / R1_, P = P_, NT = P * R2, R2P = R2 + 1;
   static constexpr int PS = R1 * R2P;
   static constexpr int NB = R1 / R2;
@@ -661,9 +668,21 @@ bool fast_s_len(Run& run, const SArgs& a) {
   return false;
 }
 
+// x-pass tile of the 3-component operator: columns per block, octant run
+// staged in LDS, minimum waves per SIMD (register budget)
+#ifndef E3_X3_TC
+#define E3_X3_TC (128 / (int)sizeof(cpx<T>))
+#endif
+#ifndef E3_X3_STAGE
+#define E3_X3_STAGE false
+#endif
+#ifndef E3_X3_WAVES
+#define E3_X3_WAVES 2
+#endif
+
 template <typename T, int L, class Run>
 bool fast_x_len(Run& run, const XArgs& a) {
-  using X3 = FastX<T, L, 3>;
+  using X3 = FastX<T, L, 3, E3_X3_TC, E3_X3_STAGE, E3_X3_WAVES>;
   using X1 = FastX<T, L, 1>;
   if (a.N[0] * 2 != L || a.L[2] % X3::TC != 0 || !a.perm) return false;
   if (a.C == 3) {

@@ -24,19 +24,24 @@ __global__ __launch_bounds__(K::NT) void k_fast_s(SArgs a, long long ncb) {
 
 // Phases 0 .. NPH-1 of a block, each a compile-time instance, barriers between.
 template <class K, int... I>
-__device__ __forceinline__ void run_phases(K& k, typename K::St& st, std::integer_sequence<int, I...>) {
-  ((k.template phase<I>(threadIdx.x, st), (I + 1 < K::NPH ? __syncthreads() : (void)0)), ...);
+__device__ __forceinline__ void run_phases(K& k, int h, typename K::St& st, std::integer_sequence<int, I...>) {
+  ((k.template phase<I>(threadIdx.x, h, st), (I + 1 < K::NPH ? __syncthreads() : (void)0)), ...);
 }
 
 template <class K, int C>
-__global__ __launch_bounds__(K::NT) void k_fast_x(XArgs a, long long ncb) {
+__global__ __launch_bounds__(K::NT, K::MIN_WAVES_PER_EU) void k_fast_x(XArgs a, long long ncb) {
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
   const long long bid = blockIdx.x;
   const int b = (int)(bid / ncb);
   K k(a, bid - (long long)b * ncb, b, lds);
   typename K::St st;
-  run_phases(k, st, std::make_integer_sequence<int, K::NPH>{});
+  const int nrow = k.rows();
+#pragma unroll 1
+  for (int h = 0; h < nrow; ++h) {
+    if (h) __syncthreads();
+    run_phases(k, h, st, std::make_integer_sequence<int, K::NPH>{});
+  }
 }
 
 template <class K>
@@ -72,17 +77,6 @@ __global__ __launch_bounds__(K::NT) void k_fast_zi(ZArgs a, long long nblk) {
   k.epilogue(threadIdx.x, st);
 }
 
-template <class X>
-struct CompOf;
-template <typename T, int L, int C>
-struct CompOf<FastX<T, L, C>> {
-  static constexpr int value = C;
-};
-template <typename T, int L, int R1, int P, int C>
-struct CompOf<FastZI<T, L, R1, P, C>> {
-  static constexpr int value = C;
-};
-
 struct HipFastRun {
   hipStream_t st;
   hipError_t err = hipSuccess;
@@ -95,7 +89,7 @@ struct HipFastRun {
   template <class K>
   void x(const XArgs& a) {
     const long long ncb = K::blocks(a);
-    hipLaunchKernelGGL((k_fast_x<K, CompOf<K>::value>), dim3((unsigned)(ncb * a.batch)), dim3(K::NT), 0, st,
+    hipLaunchKernelGGL((k_fast_x<K, K::NCOMP>), dim3((unsigned)(ncb * a.batch)), dim3(K::NT), 0, st,
                        a, ncb);
     err = hipGetLastError();
   }
@@ -108,7 +102,7 @@ struct HipFastRun {
   template <class K>
   void zi(const ZArgs& a) {
     const long long nblk = (a.P + K::P - 1) / K::P;
-    hipLaunchKernelGGL((k_fast_zi<K, CompOf<K>::value>), dim3((unsigned)(nblk * a.batch)), dim3(K::NT), 0, st,
+    hipLaunchKernelGGL((k_fast_zi<K, K::NCOMP>), dim3((unsigned)(nblk * a.batch)), dim3(K::NT), 0, st,
                        a, nblk);
     err = hipGetLastError();
   }

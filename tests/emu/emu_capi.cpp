@@ -64,13 +64,13 @@ struct EmuFastRun {
       const int b = (int)(bid / ncb);
       K k(a, bid - (long long)b * ncb, b, (cpx<T>*)mem.data());
       std::vector<typename K::St> st(K::NT);
-      phases(k, st, std::make_integer_sequence<int, K::NPH>{});
+      for (int h = 0; h < k.rows(); ++h) phases(k, h, st, std::make_integer_sequence<int, K::NPH>{});
     }
   }
   template <class K, int... I>
-  static void phases(K& k, std::vector<typename K::St>& st, std::integer_sequence<int, I...>) {
+  static void phases(K& k, int h, std::vector<typename K::St>& st, std::integer_sequence<int, I...>) {
     auto one = [&](auto ic) {
-      for (int t = 0; t < K::NT; ++t) k.template phase<decltype(ic)::value>(t, st[t]);
+      for (int t = 0; t < K::NT; ++t) k.template phase<decltype(ic)::value>(t, h, st[t]);
     };
     (one(std::integral_constant<int, I>{}), ...);
   }
