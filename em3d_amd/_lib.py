@@ -11,6 +11,19 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libem3d_hip.so")
 
+class BiCGStabParams(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("rtol", C.c_double), ("atol", C.c_double),
+                ("divergence_guard", C.c_double), ("max_operator_actions", C.c_longlong), ("chunk", C.c_int)]
+
+
+class SolverResultC(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("matvec_count", C.c_int),
+                ("history_len", C.c_int), ("true_final_residual", C.c_double)]
+
+
+STATUS = {0: "converged", 1: "nonfinite", 2: "max_operator_actions", 3: "breakdown_rho", 4: "breakdown_omega",
+          5: "breakdown_alpha", 6: "breakdown_t", 7: "divergence_guard", 8: "max_iter"}
+
 # Every symbol the header declares, with (restype, argtypes).
 _VP, _I, _D, _LL, _SZ = C.c_void_p, C.c_int, C.c_double, C.c_longlong, C.c_size_t
 SIGNATURES = {
@@ -24,6 +37,10 @@ SIGNATURES = {
     "em3d_rmatvec": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
     "em3d_plan_set_timing": (_I, [_VP, _I]),
     "em3d_plan_timing": (_I, [_VP, C.POINTER(_D), _I, C.POINTER(_I), C.POINTER(_LL)]),
+    "em3d_bicgstab_work_bytes": (_SZ, [_VP]),
+    "em3d_bicgstab": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, C.POINTER(BiCGStabParams), _VP, _VP, _I,
+                           C.POINTER(SolverResultC), _VP]),
+    "em3d_convert_axpy": (_I, [_LL, _I, _VP, _D, _D, _I, _VP, _I, _VP]),
     "em3d_reduce_work_bytes": (_SZ, []),
     "em3d_dotc": (_I, [_I, _LL, _VP, _VP, _VP, _VP, _VP]),
     "em3d_axpbypcz": (_I, [_I, _LL, _D, _D, _VP, _D, _D, _VP, _D, _D, _VP, _VP, _VP]),

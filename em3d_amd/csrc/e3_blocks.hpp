@@ -166,6 +166,7 @@ struct ZArgs {
   int batch;
   int perm;                    // doubled axis in mirror-interleaved order (output of a
                                // forward pass, input of an inverse pass)
+  const int* skip;             // device flag: when non-zero the whole pass is a no-op
 };
 
 template <typename T>
@@ -240,6 +241,7 @@ struct SArgs {
   AxisPlan plan;
   const void* tw;
   int inv;
+  const int* skip;             // device flag: when non-zero the whole pass is a no-op
 };
 
 template <typename T>
@@ -293,6 +295,7 @@ struct XArgs {
   int adj;
   int batch;
   int perm;            // z axis of W2 in mirror-interleaved order
+  const int* skip;     // device flag: when non-zero the whole pass is a no-op
 };
 
 // ---------------------------------------------------------------------------

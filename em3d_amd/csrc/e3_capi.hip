@@ -15,13 +15,12 @@
 #include "../../include/em3d_hip.h"
 #include "e3_error.hpp"
 #include "e3_fast_launch.hpp"
+#include "e3_plan.hpp"
 #include "e3_pipeline.hpp"
 
 using namespace e3;
 
 namespace {
-
-constexpr int kMaxTimedLaunches = 8;
 
 int fail(const std::string& msg) { return set_error(msg); }
 
@@ -37,6 +36,7 @@ extern "C" __shared__ double2 e3_smem[];
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_zpass(ZArgs a, long long nblk) {
+  if (a.skip && *a.skip) return;
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(e3_smem);
   const long long bid = blockIdx.x;
   const int b = (int)(bid / nblk);
@@ -52,6 +52,7 @@ __global__ __launch_bounds__(256) void k_zpass(ZArgs a, long long nblk) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_spass(SArgs a, long long ncb) {
+  if (a.skip && *a.skip) return;
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(e3_smem);
   const long long bid = blockIdx.x;
   const long long q = bid / ncb;
@@ -67,6 +68,7 @@ __global__ __launch_bounds__(256) void k_spass(SArgs a, long long ncb) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_xpass(XArgs a, long long ncb) {
+  if (a.skip && *a.skip) return;
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(e3_smem);
   const long long bid = blockIdx.x;
   const int b = (int)(bid / ncb);
@@ -177,18 +179,30 @@ void set_lds_limits_once() {
 
 }  // namespace
 
-struct em3d_plan_s {
-  Plan p;
-  void* tw_dev = nullptr;
-  int device = 0;
-  // per-pass timing (em3d_plan_set_timing): events around the launches of
-  // each apply call and the accumulated elapsed milliseconds per launch slot
-  bool timing = false;
-  hipEvent_t ev[kMaxTimedLaunches + 1] = {};
-  double ms[kMaxTimedLaunches] = {};
-  long long calls = 0;
-  int nlaunch = 0;
-};
+int e3::apply_plan(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
+                   void* work, hipStream_t stream, bool adjoint, const char* name, const int* skip) {
+  if (!plan || !khat || !u || !y || !work) return fail(std::string(name) + ": null argument");
+  if (chi_kind < CHI_NONE || chi_kind > CHI_FULL) return fail(std::string(name) + ": bad chi_kind");
+  if (chi_kind != CHI_NONE && !chi) return fail(std::string(name) + ": null chi");
+  if (plan->p.C == 1 && chi_kind > CHI_ISO) return fail(std::string(name) + ": acoustic contrast must be scalar");
+  if (u == y) return fail(std::string(name) + ": output must not alias the input");
+  HipLauncher go{stream};
+  if (plan->timing) go.ev = plan->ev;
+  run_apply(go, plan->p, khat, chi, chi_kind, u, y, work, adjoint, skip);
+  if (plan->timing && go.launches <= kMaxTimedLaunches) {
+    go.note(hipEventRecord(plan->ev[go.launches], stream));
+    go.note(hipEventSynchronize(plan->ev[go.launches]));
+    for (int i = 0; i < go.launches; ++i) {
+      float t = 0.f;
+      go.note(hipEventElapsedTime(&t, plan->ev[i], plan->ev[i + 1]));
+      plan->ms[i] += t;
+    }
+    plan->nlaunch = go.launches;
+    plan->calls += 1;
+  }
+  if (go.err != hipSuccess) return hip_check(go.err, name);
+  return 0;
+}
 
 extern "C" {
 
@@ -310,39 +324,14 @@ int em3d_kernel_build(em3d_plan_t plan, double k0, double hx, double hy, double 
   return 0;
 }
 
-static int apply(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
-                 void* work, hipStream_t stream, bool adjoint, const char* name) {
-  if (!plan || !khat || !u || !y || !work) return fail(std::string(name) + ": null argument");
-  if (chi_kind < CHI_NONE || chi_kind > CHI_FULL) return fail(std::string(name) + ": bad chi_kind");
-  if (chi_kind != CHI_NONE && !chi) return fail(std::string(name) + ": null chi");
-  if (plan->p.C == 1 && chi_kind > CHI_ISO) return fail(std::string(name) + ": acoustic contrast must be scalar");
-  if (u == y) return fail(std::string(name) + ": output must not alias the input");
-  HipLauncher go{stream};
-  if (plan->timing) go.ev = plan->ev;
-  run_apply(go, plan->p, khat, chi, chi_kind, u, y, work, adjoint);
-  if (plan->timing && go.launches <= kMaxTimedLaunches) {
-    go.note(hipEventRecord(plan->ev[go.launches], stream));
-    go.note(hipEventSynchronize(plan->ev[go.launches]));
-    for (int i = 0; i < go.launches; ++i) {
-      float t = 0.f;
-      go.note(hipEventElapsedTime(&t, plan->ev[i], plan->ev[i + 1]));
-      plan->ms[i] += t;
-    }
-    plan->nlaunch = go.launches;
-    plan->calls += 1;
-  }
-  if (go.err != hipSuccess) return hip_check(go.err, name);
-  return 0;
-}
-
 int em3d_matvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
                 void* work, hipStream_t stream) {
-  return apply(plan, khat, chi, chi_kind, u, y, work, stream, false, "em3d_matvec");
+  return e3::apply_plan(plan, khat, chi, chi_kind, u, y, work, stream, false, "em3d_matvec", nullptr);
 }
 
 int em3d_rmatvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* v, void* y,
                  void* work, hipStream_t stream) {
-  return apply(plan, khat, chi, chi_kind, v, y, work, stream, true, "em3d_rmatvec");
+  return e3::apply_plan(plan, khat, chi, chi_kind, v, y, work, stream, true, "em3d_rmatvec", nullptr);
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@ namespace e3 {
 
 template <class K>
 __global__ __launch_bounds__(K::NT) void k_fast_s(SArgs a, long long ncb) {
+  if (a.skip && *a.skip) return;
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
   const long long bid = blockIdx.x;
@@ -30,6 +31,7 @@ __device__ __forceinline__ void run_phases(K& k, int h, typename K::St& st, std:
 
 template <class K, int C>
 __global__ __launch_bounds__(K::NT, K::MIN_WAVES_PER_EU) void k_fast_x(XArgs a, long long ncb) {
+  if (a.skip && *a.skip) return;
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
   const long long bid = blockIdx.x;
@@ -46,6 +48,7 @@ __global__ __launch_bounds__(K::NT, K::MIN_WAVES_PER_EU) void k_fast_x(XArgs a, 
 
 template <class K>
 __global__ __launch_bounds__(K::NT) void k_fast_zf(ZArgs a, long long nblk) {
+  if (a.skip && *a.skip) return;
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
   const long long bid = blockIdx.x;
@@ -61,6 +64,7 @@ __global__ __launch_bounds__(K::NT) void k_fast_zf(ZArgs a, long long nblk) {
 
 template <class K, int C>
 __global__ __launch_bounds__(K::NT) void k_fast_zi(ZArgs a, long long nblk) {
+  if (a.skip && *a.skip) return;
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
   const long long bid = blockIdx.x;

@@ -70,6 +70,19 @@ __global__ __launch_bounds__(256) void k_axpbypcz(long long n, cpx<T> a, const c
   }
 }
 
+// dst = beta dst + a src, converting between complex64 and complex128
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void k_convert_axpy(long long n, const cpx<S>* src, cpx<double> a, cpx<D>* dst,
+                                                      int accumulate) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const cpx<double> x = mk((double)src[i].re, (double)src[i].im);
+    cpx<double> y = a * x;
+    if (accumulate) y = y + mk((double)dst[i].re, (double)dst[i].im);
+    dst[i] = mk((D)y.re, (D)y.im);
+  }
+}
+
 int kfail(const std::string& m) { return set_error(m); }
 
 }  // namespace
@@ -114,6 +127,30 @@ int em3d_axpbypcz(int precision, long long n, double a_re, double a_im, const vo
                        mk((float)c_re, (float)c_im), (const cpx<float>*)z, (cpx<float>*)out);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return kfail(std::string("em3d_axpbypcz: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int em3d_convert_axpy(long long n, int src_precision, const void* src, double a_re, double a_im, int dst_precision,
+                      void* dst, int accumulate, hipStream_t stream) {
+  if (!src || !dst) return kfail("em3d_convert_axpy: null argument");
+  if (n <= 0) return 0;
+  const long long want = (n + 255) / 256;
+  const unsigned nb = (unsigned)(want > 65536 ? 65536 : want);
+  const cpx<double> a = mk(a_re, a_im);
+  if (src_precision == 1 && dst_precision == 1)
+    hipLaunchKernelGGL((k_convert_axpy<double, double>), dim3(nb), dim3(256), 0, stream, n, (const cpx<double>*)src, a,
+                       (cpx<double>*)dst, accumulate);
+  else if (src_precision == 1)
+    hipLaunchKernelGGL((k_convert_axpy<double, float>), dim3(nb), dim3(256), 0, stream, n, (const cpx<double>*)src, a,
+                       (cpx<float>*)dst, accumulate);
+  else if (dst_precision == 1)
+    hipLaunchKernelGGL((k_convert_axpy<float, double>), dim3(nb), dim3(256), 0, stream, n, (const cpx<float>*)src, a,
+                       (cpx<double>*)dst, accumulate);
+  else
+    hipLaunchKernelGGL((k_convert_axpy<float, float>), dim3(nb), dim3(256), 0, stream, n, (const cpx<float>*)src, a,
+                       (cpx<float>*)dst, accumulate);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return kfail(std::string("em3d_convert_axpy: ") + hipGetErrorString(e));
   return 0;
 }
 

@@ -103,7 +103,7 @@ void s_pass(Launcher& go, int prec, const void* tw, const AxisPlan& ax, SArgs s)
 // y = A u (adjoint = false) or y = A^H u (adjoint = true) for all batch RHS.
 template <class Launcher>
 void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, int chi_kind,
-               const void* u, void* y, void* work, bool adjoint) {
+               const void* u, void* y, void* work, bool adjoint, const int* skip = nullptr) {
   const int prec = p.prec;
   const size_t es = elem_bytes(prec);
   void* W1 = work;
@@ -123,7 +123,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.out_ps = Lz; z.out_cs = P * Lz; z.out_bs = C * P * Lz;
   z.n_in = Nz; z.n_out = Lz; z.inv = 0;
   z.pro = adjoint ? PRO_COPY : PRO_CHI; z.epi = EPI_STORE;
-  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.perm = 1;
+  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.perm = 1; z.skip = skip;
   z_pass(go, p, prec, twz, p.ax[2], z);
 
   // P2: y forward (Ny live rows -> Ly)
@@ -131,7 +131,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   std::memset(&s, 0, sizeof(s));
   s.in = W1; s.out = W2; s.Q = (long long)p.batch * C * Nx; s.W = Lz;
   s.in_qs = (long long)Ny * Lz; s.out_qs = (long long)Ly * Lz;
-  s.n_in = Ny; s.n_out = Ly; s.inv = 0;
+  s.n_in = Ny; s.n_out = Ly; s.inv = 0; s.skip = skip;
   s_pass(go, prec, twy, p.ax[1], s);
 
   // P3: x forward, spectral multiply, x inverse (Nx rows kept), in place
@@ -139,7 +139,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   std::memset(&x, 0, sizeof(x));
   x.data = W2; x.C = p.C; x.W = (long long)Ly * Lz; x.cs = (long long)Nx * x.W; x.bs = C * x.cs;
   for (int a = 0; a < 3; ++a) { x.N[a] = p.N[a]; x.L[a] = p.L[a]; }
-  x.plan = p.ax[0]; x.tw = twx; x.khat = khat; x.adj = adjoint ? 1 : 0; x.batch = p.batch; x.perm = 1;
+  x.plan = p.ax[0]; x.tw = twx; x.khat = khat; x.adj = adjoint ? 1 : 0; x.batch = p.batch; x.perm = 1; x.skip = skip;
   x.ps = Lx + 1;
   x.T = tile_count(lds_bytes(p.C, x.ps, prec), 16);
   if (x.T > x.W) x.T = (int)x.W;
@@ -157,7 +157,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.out_ps = Nz; z.out_cs = P * Nz; z.out_bs = C * P * Nz;
   z.n_in = Lz; z.n_out = Nz; z.inv = 1;
   z.pro = PRO_COPY; z.epi = adjoint ? EPI_SUB_CHIH : EPI_SUB;
-  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.u = u; z.perm = 1;
+  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.u = u; z.perm = 1; z.skip = skip;
   z_pass(go, p, prec, twz, p.ax[2], z);
 }
 

@@ -283,6 +283,40 @@ class _FieldApplier:
             raise ValueError(f"field shape {tuple(t.shape)} != expected {fs} (or (batch,) + {fs})")
         return t.contiguous(), batch
 
+    def _device_bicgstab(self, rhs, cfg, khat, chi_kind, chi):
+        """Device-resident BiCGStab of libem3d_hip on this operator (batch-1 field)."""
+        import torch
+
+        from .solvers.base import SolverResult
+        t, batch = self._prepare_input(rhs)
+        if batch != 1:
+            raise ValueError("device BiCGStab solves one right-hand side")
+        be = self._be
+        plan = HipPlan.get(self._N, self._ncomp, 1, be.precision, be.device_index)
+        lib = _lib.lib()
+        sw = torch.empty(int(lib.em3d_bicgstab_work_bytes(plan.handle)), dtype=torch.uint8, device=be.torch_device)
+        cap = int(cfg.max_iter) + 2
+        hist = torch.zeros(cap, dtype=torch.float64, device=be.torch_device)
+        acts = torch.zeros(cap, dtype=torch.int64, device=be.torch_device)
+        u = torch.empty_like(t)
+        prm = _lib.BiCGStabParams(int(cfg.max_iter), float(cfg.rtol), float(cfg.atol),
+                                  -1.0 if cfg.divergence_guard is None else float(cfg.divergence_guard),
+                                  -1 if cfg.max_operator_actions is None else int(cfg.max_operator_actions),
+                                  int(getattr(cfg, "chunk", 4) or 4))
+        res = _lib.SolverResultC()
+        _lib.check(lib.em3d_bicgstab(plan.handle, khat.data_ptr(), chi.data_ptr() if chi is not None else None,
+                                     chi_kind, t.data_ptr(), u.data_ptr(), self._workspace(plan).data_ptr(),
+                                     sw.data_ptr(), C.byref(prm), hist.data_ptr(), acts.data_ptr(), cap,
+                                     C.byref(res), _lib.stream_handle(be.torch_device)), "em3d_bicgstab")
+        n = int(res.history_len)
+        history = [float(x) for x in hist[:n].cpu().tolist()]
+        counts = [int(x) for x in acts[:n].cpu().tolist()]
+        status = _lib.STATUS[int(res.status)]
+        return SolverResult(u=u, iterations=int(res.iterations), residual_history=history,
+                            converged=status == "converged", matvec_count=int(res.matvec_count), rmatvec_count=0,
+                            residual_action_counts=counts, status=status,
+                            true_final_residual=float(history[-1]) if history else float("inf"))
+
     def _apply(self, field, khat, chi_kind, chi, adjoint: bool):
         import torch
         t, batch = self._prepare_input(field)
@@ -330,6 +364,10 @@ class Operator(_FieldApplier):
 
     def matvec(self, field):
         return self._apply(field, self._prepared_kernel.khat_octant, self._chi_kind, self._chi, False)
+
+    def device_bicgstab(self, rhs, cfg):
+        """BiCGStab with all iteration scalars on the device (libem3d_hip)."""
+        return self._device_bicgstab(rhs, cfg, self._prepared_kernel.khat_octant, self._chi_kind, self._chi)
 
     def rmatvec(self, field):
         if not self._prepared_kernel.supports_adjoint:

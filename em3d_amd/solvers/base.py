@@ -15,6 +15,7 @@ class SolverConfig:
     log: bool = False
     mu: Optional[complex] = None
     radius: Optional[float] = None
+    chunk: int = 4          # device solvers: iterations enqueued between host checks
 
     def require_gamma(self) -> None:
         if self.mu is None or self.radius is None:

@@ -24,6 +24,15 @@ class BiCGStab:
         self.cfg = config
 
     def solve(self, operator, rhs) -> SolverResult:
+        """Solve ``operator u = rhs``.  em3d_amd operators run the device-resident
+        solver of libem3d_hip; any other operator object (the reference's duck-typed
+        protocol: ``backend`` and ``matvec``) runs the host-driven loop below."""
+        fused = getattr(operator, "device_bicgstab", None)
+        if fused is not None and getattr(rhs, "dim", lambda: 0)() == len(getattr(operator, "_field_shape")()):
+            return fused(rhs, self.cfg)
+        return self.solve_host(operator, rhs)
+
+    def solve_host(self, operator, rhs) -> SolverResult:
         cfg = self.cfg
         rhs = rhs.contiguous()
         rhs_norm = norm(rhs)

@@ -88,6 +88,40 @@ int em3d_rmatvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_ki
 int em3d_plan_set_timing(em3d_plan_t plan, int enable);
 int em3d_plan_timing(em3d_plan_t plan, double* ms_per_launch, int max_launches, int* nlaunch, long long* calls);
 
+/* Device-resident BiCGStab on the operator of a batch-1 plan
+ * (BiCGStab.solve, solvers/bicgstab.py:12-211, with SolverConfig fields of
+ * solvers/base.py:7-25).  Same recurrence, early exit on ||s||, statuses and
+ * history semantics as the reference; every scalar stays on the device and
+ * the host reads the solver state back once per `chunk` iterations.
+ * u (out) and rhs are device fields of the plan's shape.  `work` is the
+ * plan's apply workspace, `solver_work` holds em3d_bicgstab_work_bytes().
+ * history / action_counts are DEVICE arrays of history_cap entries that
+ * receive the relative residuals and the action count of each entry.
+ * Negative divergence_guard / max_operator_actions mean "not set". */
+typedef struct {
+  int max_iter;
+  double rtol, atol;
+  double divergence_guard;
+  long long max_operator_actions;
+  int chunk;
+} em3d_bicgstab_params;
+
+/* status: 0 converged, 1 nonfinite, 2 max_operator_actions, 3 breakdown_rho,
+ * 4 breakdown_omega, 5 breakdown_alpha, 6 breakdown_t, 7 divergence_guard,
+ * 8 max_iter */
+typedef struct {
+  int status;
+  int iterations;
+  int matvec_count;
+  int history_len;
+  double true_final_residual;
+} em3d_solver_result;
+
+size_t em3d_bicgstab_work_bytes(em3d_plan_t plan);
+int em3d_bicgstab(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* rhs, void* u,
+                  void* work, void* solver_work, const em3d_bicgstab_params* params, double* history,
+                  long long* action_counts, int history_cap, em3d_solver_result* result, hipStream_t stream);
+
 /* Bytes of the reduction scratch used by em3d_dotc. */
 size_t em3d_reduce_work_bytes(void);
 
@@ -103,6 +137,13 @@ int em3d_dotc(int precision, long long n, const void* x, const void* y, void* ou
 int em3d_axpbypcz(int precision, long long n, double a_re, double a_im, const void* x, double b_re,
                   double b_im, const void* y, double c_re, double c_im, const void* z, void* out,
                   hipStream_t stream);
+
+/* dst = a src (+ dst when accumulate != 0), converting between complex64
+ * (precision 0) and complex128 (1): the precision changes of the mixed
+ * precision refinement, which the reference has no counterpart for (its
+ * solvers run in one precision, solvers/bicgstab.py:12-211). */
+int em3d_convert_axpy(long long n, int src_precision, const void* src, double a_re, double a_im, int dst_precision,
+                      void* dst, int accumulate, hipStream_t stream);
 
 #ifdef __cplusplus
 }
