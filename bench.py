@@ -50,6 +50,60 @@ def make_problem(em, be, N, k0):
     return em.Problem(grid=grid, eps_tensor=chi, wave=wave, k0=k0, volume=grid.dv * N ** 3), mask
 
 
+def _true_residual(op64, u, f):
+    import torch
+    r = op64.matvec(u) - f
+    return float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(f))
+
+
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def solve_section(em, N, k0s):
+    """Time to true rtol 1e-8 at N^3 for each k0: plain complex128 BiCGStab
+    vs complex64 BiCGStab with FP64 refinement.  Wall clock from solve()
+    entry to exit on a synchronised device; the complex64 spectrum build of
+    the refinement is one-time setup, reported apart."""
+    import torch
+    be64 = em.Backend.hip(em.Precision.DOUBLE)
+    cfg = em.SolverConfig(max_iter=1000, rtol=1e-8)
+    out = {}
+    for k0 in k0s:
+        prob, _ = make_problem(em, be64, N, k0)
+        op = em.Operator(prob)
+        f = prob.wave
+        _, t_mv = _timed(lambda: [op.matvec(f) for _ in range(10)])
+        _, t_setup = _timed(op.single_precision_companion)
+        op.single_precision_companion().matvec(f.to(torch.complex64))  # warm the complex64 plan
+        plain, t_plain = _timed(lambda: em.BiCGStab(cfg).solve(op, f))
+        mixed, t_mixed = _timed(lambda: em.MixedBiCGStab(cfg).solve(op, f))
+        rec = {
+            "fp64_matvec_ms": round(t_mv / 10 * 1e3, 4),
+            "fp64_bicgstab": {"seconds": round(t_plain, 4), "iterations": plain.iterations,
+                              "matvecs": plain.matvec_count, "status": plain.status,
+                              "true_residual": _true_residual(op, plain.u, f)},
+            "mixed_bicgstab": {"seconds": round(t_mixed, 4), "iterations": mixed.iterations,
+                               "fp32_matvecs": mixed.extras["fp32_matvecs"],
+                               "fp64_matvecs": mixed.extras["fp64_matvecs"], "outer": mixed.extras["outer"],
+                               "status": mixed.status, "true_residual": _true_residual(op, mixed.u, f),
+                               "setup_seconds": round(t_setup, 4)},
+            "solution_rel_diff": float(torch.linalg.vector_norm(mixed.u - plain.u) / torch.linalg.vector_norm(plain.u)),
+            "speedup_mixed_vs_fp64": round(t_plain / t_mixed, 3),
+        }
+        log(f"[bench] k0={k0}: fp64 {t_plain:.3f} s ({plain.iterations} it, {plain.status}), "
+            f"mixed {t_mixed:.3f} s ({mixed.iterations} it, {mixed.status}), x{t_plain / t_mixed:.2f}")
+        out[f"k0={k0:g}"] = rec
+        del op, prob, plain, mixed
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(N, reps, k0):
     """Oracle (NumPy, complex64 as the reference's SINGLE path) matvec at the
     same size; the spectrum is a random tensor of the reference's dense
@@ -85,13 +139,15 @@ def main():
     ap.add_argument("--k0", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--k0s", type=float, nargs="*", default=[4.0, 8.0, 12.0],
+                    help="wave numbers of the time-to-rtol solves")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run 3 plain device copies of 512 MiB (read 512 MiB, write 512 MiB each), "
                          "the calibration kernel for HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the operator shards by right-hand side / material, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: the path runs on one "
+              "GPU (right-hand sides and materials would shard one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -153,28 +209,7 @@ def main():
 
     solve = None
     if not args.no_solve:
-        solve = {}
-        for name, prec in (("fp64_bicgstab", em.Precision.DOUBLE), ("fp32_bicgstab", em.Precision.SINGLE)):
-            be = em.Backend.hip(prec)
-            p2, _ = make_problem(em, be, N, k0)
-            o2 = em.Operator(p2)
-            cfg = em.SolverConfig(max_iter=500, rtol=1e-8 if prec is em.Precision.DOUBLE else 1e-5)
-            o2.matvec(p2.wave)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            res = em.BiCGStab(cfg).solve(o2, p2.wave)
-            torch.cuda.synchronize()
-            dt = time.perf_counter() - t0
-            if prec is em.Precision.DOUBLE:
-                r = o2.matvec(res.u) - p2.wave
-                true_res = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(p2.wave))
-            else:
-                true_res = float(res.residual_history[-1])
-            solve[name] = {"seconds": round(dt, 4), "iterations": res.iterations, "status": res.status,
-                           "rtol": cfg.rtol, "final_residual": true_res}
-            log(f"[bench] {name}: {dt:.3f} s, {res.iterations} it, {res.status}, res {true_res:.2e}")
-            del o2, p2, res
-            torch.cuda.empty_cache()
+        solve = solve_section(em, N, args.k0s)
 
     cpu = None
     if not args.no_cpu:

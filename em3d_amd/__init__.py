@@ -14,12 +14,12 @@ from .grid import Grid
 from .operator import Operator, PreparedEMKernel
 from .problem import Problem
 from .refraction import apply_refraction, cylinder_refraction, ellipsis_refraction, step_refraction
-from .solvers import SIM, BaseSolver, BiCGStab, SolverConfig, SolverResult, TwoStep
+from .solvers import SIM, BaseSolver, BiCGStab, MixedBiCGStab, SolverConfig, SolverResult, TwoStep
 from .wave import flat_wave_vec
 
 __version__ = "0.1.0"
 
 __all__ = ["Backend", "Precision", "Grid", "Problem", "Operator", "PreparedEMKernel", "apply_refraction",
            "cylinder_refraction", "ellipsis_refraction", "step_refraction", "flat_wave_vec", "geometry",
-           "acoustics", "BaseSolver", "BiCGStab", "SIM", "SolverConfig", "SolverResult", "TwoStep",
+           "acoustics", "BaseSolver", "BiCGStab", "MixedBiCGStab", "SIM", "SolverConfig", "SolverResult", "TwoStep",
            "__version__"]

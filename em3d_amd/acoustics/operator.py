@@ -12,9 +12,12 @@ class AcousticOperator(_FieldApplier):
         self.problem = problem
         be = problem.grid.backend
         require_hip(be)
-        self._init_applier(be, problem.grid.N, 1)
+        self._init_applier(be, problem.grid.N, 1, problem.grid.L, problem.k0)
         self._K_hat = prep_coeffs_acoustic(problem.grid, k0=problem.k0)
         self._chi = problem.chi.contiguous()
+
+    def _spectrum(self):
+        return self._K_hat, CHI_ISO, self._chi
 
     @property
     def backend(self):

@@ -249,11 +249,35 @@ def pack_contrast(chi):
 class _FieldApplier:
     """Shared matvec/rmatvec plumbing: shape checks, plans and workspaces."""
 
-    def _init_applier(self, be, N, ncomp):
+    def _init_applier(self, be, N, ncomp, L=None, k0=None):
         self._be = be
         self._N = tuple(int(n) for n in N)
         self._ncomp = ncomp
+        self._L = None if L is None else tuple(float(v) for v in L)
+        self._k0 = None if k0 is None else float(k0)
         self._work: dict[int, object] = {}
+        self._companion = None
+
+    def _spectrum(self):
+        """(octant spectrum, chi kind, packed chi) this operator applies."""
+        raise NotImplementedError
+
+    def single_precision_companion(self) -> "_SingleCompanion":
+        """The same operator in complex64: spectrum rebuilt in FP64 and cast,
+        contrast cast from this operator's.  Built once and cached."""
+        import torch
+        if self._companion is None:
+            if self._be.precision is Precision.SINGLE:
+                self._companion = self
+            else:
+                from .backend import Backend
+                _, kind, chi = self._spectrum()
+                be32 = Backend.hip(Precision.SINGLE, self._be.device_index)
+                khat = build_octant_spectrum(self._N, self._L, self._k0, self._ncomp, Precision.SINGLE,
+                                             self._be.device_index)
+                chi32 = None if chi is None else chi.to(torch.complex64).contiguous()
+                self._companion = _SingleCompanion(be32, self._N, self._ncomp, self._L, self._k0, khat, kind, chi32)
+        return self._companion
 
     def _field_shape(self):
         return ((3,) if self._ncomp == 3 else ()) + self._N
@@ -330,6 +354,30 @@ class _FieldApplier:
         return y
 
 
+class _SingleCompanion(_FieldApplier):
+    """complex64 twin of an FP64 operator (inner solves of the mixed-precision refinement)."""
+
+    def __init__(self, be, N, ncomp, L, k0, khat, chi_kind, chi):
+        self._init_applier(be, N, ncomp, L, k0)
+        self._khat, self._chi_kind, self._chi = khat, chi_kind, chi
+
+    @property
+    def backend(self):
+        return self._be
+
+    def _spectrum(self):
+        return self._khat, self._chi_kind, self._chi
+
+    def matvec(self, field):
+        return self._apply(field, self._khat, self._chi_kind, self._chi, False)
+
+    def rmatvec(self, field):
+        return self._apply(field, self._khat, self._chi_kind, self._chi, True)
+
+    def device_bicgstab(self, rhs, cfg):
+        return self._device_bicgstab(rhs, cfg, self._khat, self._chi_kind, self._chi)
+
+
 class Operator(_FieldApplier):
     """``A u = u - B(chi u)`` on the HIP backend (operator.py:320-428)."""
 
@@ -342,8 +390,11 @@ class Operator(_FieldApplier):
         else:
             prepared_kernel.validate(grid, k=problem.k0)
         self._prepared_kernel = prepared_kernel
-        self._init_applier(grid.backend, grid.N, 3)
+        self._init_applier(grid.backend, grid.N, 3, grid.L, problem.k0)
         self._chi_kind, self._chi = pack_contrast(problem.eps_tensor)
+
+    def _spectrum(self):
+        return self._prepared_kernel.khat_octant, self._chi_kind, self._chi
 
     @property
     def backend(self):

@@ -1,6 +1,7 @@
 from .base import BaseSolver, SolverConfig, SolverResult
 from .bicgstab import BiCGStab
+from .refine import MixedBiCGStab
 from .sim import SIM
 from .twostep import TwoStep
 
-__all__ = ["BaseSolver", "SolverConfig", "SolverResult", "SIM", "BiCGStab", "TwoStep"]
+__all__ = ["BaseSolver", "SolverConfig", "SolverResult", "SIM", "BiCGStab", "MixedBiCGStab", "TwoStep"]

@@ -35,6 +35,7 @@ class SolverResult:
     residual_action_counts: List[int] = field(default_factory=list)
     status: str = "unknown"
     true_final_residual: float | None = None
+    extras: dict | None = None      # solver-specific details (refinement steps, precisions)
 
     @property
     def operator_action_count(self) -> int:

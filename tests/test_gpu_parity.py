@@ -221,3 +221,50 @@ def test_forced_statuses_on_gpu(em):
         em.Operator(prob, prepared_kernel=pk).rmatvec(prob.wave)
     with pytest.raises(ValueError, match="wave number"):
         em.Operator(prob, prepared_kernel=em.PreparedEMKernel.build(grid, k=2.0))
+
+
+def _config2(em, be, N, k):
+    """Anisotropic absorbing ellipsoid: radii (0.3, 0.2, 0.15), chi = diag(1.5+0.1i, 2.0+0.2i, 1.0)."""
+    return _problem(em, be, N, (1.0, 1.0, 1.0), k, _chi("diag", N))
+
+
+def test_mixed_refinement_reaches_fp64_tolerance(em):
+    """complex64 inner BiCGStab + FP64 refinement: true residual (recomputed
+    here with the FP64 operator) below 1e-8 and the solution within 1e-6 of
+    the oracle's complex128 BiCGStab solution."""
+    from oracle import solvers
+    N, k = (16, 16, 16), 8.0
+    _say("[mixed] 16^3 config-2 ellipsoid k0=8")
+    be = _be(em, "double")
+    prob = _config2(em, be, N, k)
+    op = em.Operator(prob)
+    res = em.MixedBiCGStab(em.SolverConfig(max_iter=200, rtol=1e-8)).solve(op, prob.wave)
+    assert res.status == "converged", res.extras
+    f = prob.wave
+    true = float(np.linalg.norm(_host(op.matvec(res.u) - f)) / np.linalg.norm(_host(f)))
+    assert true < 1e-8 and abs(true - res.true_final_residual) <= 1e-12 + 1e-6 * true
+    assert res.extras["fp32_matvecs"] > res.extras["fp64_matvecs"] >= 2
+    chi = _chi("diag", N)
+    ref = solvers.bicgstab(solvers.EMOperator(N, (1, 1, 1), k, chi), _host(f), solvers.Config(max_iter=200, rtol=1e-10))
+    assert ref.status == "converged"
+    assert _rel(_host(res.u), ref.u) < 1e-6
+
+
+def test_mixed_refinement_fallback_and_errors(em):
+    """A one-iteration inner solve cannot halve the residual: the FP64
+    BiCGStab fallback takes over and the result still meets the tolerance."""
+    N, k = (8, 8, 8), 4.0
+    _say("[mixed] 8^3 forced fallback")
+    be = _be(em, "double")
+    prob = _config2(em, be, N, k)
+    op = em.Operator(prob)
+    solver = em.MixedBiCGStab(em.SolverConfig(max_iter=200, rtol=1e-9), inner_max_iter=1, stagnation=1e-3)
+    res = solver.solve(op, prob.wave)
+    assert res.status == "converged" and res.extras["fallback"] == "converged"
+    true = float(np.linalg.norm(_host(op.matvec(res.u) - prob.wave)) / np.linalg.norm(_host(prob.wave)))
+    assert true < 1e-9
+    z = em.MixedBiCGStab(em.SolverConfig()).solve(op, be.zeros((3,) + N, "complex"))
+    assert z.status == "converged" and z.matvec_count == 0
+    be32 = _be(em, "single")
+    with pytest.raises(ValueError, match="FP64"):
+        em.MixedBiCGStab(em.SolverConfig()).solve(em.Operator(_config2(em, be32, N, k)), prob.wave)
