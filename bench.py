@@ -104,6 +104,22 @@ def solve_section(em, N, k0s):
     return out
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "em_matvec_128_c64_hbm_traffic.txt")
+
+
+def measured_traffic(kernel, N, n_chi, es):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), for the workload it was
+    measured on (128^3, diagonal chi, complex64); None otherwise."""
+    if (N, n_chi, es) != (128, 3, 8) or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
+    return None
+
+
 def cpu_baseline(N, reps, k0):
     """Oracle (NumPy, complex64 as the reference's SINGLE path) matvec at the
     same size; the spectrum is a random tensor of the reference's dense
@@ -239,7 +255,8 @@ def main():
         "per_launch_ms": {PASS_NAMES[i]: round(x, 5) for i, x in enumerate(per_launch)},
         "roofline": {"bound": "hbm", "kernel": PASS_NAMES[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic(PASS_NAMES[dom], N, n_chi, es),
+                     "algorithmic_bytes": pb[dom]},
         "solve": solve,
         "cpu_baseline": cpu,
     }
