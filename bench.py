@@ -120,6 +120,46 @@ def measured_traffic(kernel, N, n_chi, es):
     return None
 
 
+def batch_section(em, N, n_dirs, k0):
+    """Many-RHS workload: sphere (radius 0.3, eps_r 4) at N^3, n_dirs seeded
+    incident directions x 2 polarisations generated on the device, complex64.
+    Times one batched matvec over all RHS and the batched device BiCGStab to
+    rtol 1e-6 against the same RHS solved one after another."""
+    import torch
+    from em3d_amd.sources import incident_set, plane_waves
+    be = em.Backend.hip(em.Precision.SINGLE)
+    grid = em.Grid(N=(N, N, N), L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+    geo = em.geometry
+    chi = geo.ellipsoids_contrast(grid, [(geo.Ellipsoid(center=(0, 0, 0), radii=(0.3,) * 3), 4.0)])
+    o, a = incident_set(n_dirs, seed=4)
+    rhs, t_gen = _timed(lambda: plane_waves(grid, k=k0, orients=o, amplitudes=a))
+    B = rhs.shape[0]
+    op = em.Operator(em.Problem(grid=grid, eps_tensor=chi, wave=rhs[0], k0=k0, volume=1.0))
+    op.matvec(rhs)
+    _, t_mv = _timed(lambda: [op.matvec(rhs) for _ in range(3)])
+    cfg = em.SolverConfig(max_iter=500, rtol=1e-6)
+    batch, t_batch = _timed(lambda: em.BiCGStab(cfg).solve_batch(op, rhs))
+    op.device_bicgstab(rhs[0].contiguous(), cfg)
+    seq, t_seq = _timed(lambda: [em.BiCGStab(cfg).solve(op, rhs[b].contiguous()) for b in range(B)])
+    its = [r.iterations for r in batch]
+    same = all(r.iterations == q.iterations and r.status == q.status for r, q in zip(batch, seq))
+    M = N ** 3
+    per_rhs_bytes = (81 + 1 + 6 * ((N + 1) / N) ** 3) * M * 8
+    ms = t_mv / 3 * 1e3
+    out = {"workload": f"EM sphere r=0.3 eps_r=4 at {N}^3, k0={k0}, {B} RHS ({n_dirs} directions x 2 "
+                       "polarisations, default_rng(4)), complex64",
+           "rhs_generation_ms": round(t_gen * 1e3, 3),
+           "batched_matvec_ms": round(ms, 3), "rhs_matvecs_per_s": round(B / (ms * 1e-3), 1),
+           "batched_matvec_hbm_gbs": round(B * per_rhs_bytes / (ms * 1e-3) / 1e9, 1),
+           "batched_solve_s": round(t_batch, 4), "sequential_solve_s": round(t_seq, 4),
+           "solves_per_s_batched": round(B / t_batch, 2), "speedup_batched_vs_sequential": round(t_seq / t_batch, 3),
+           "iterations_min_max": [min(its), max(its)],
+           "converged": sum(r.status == "converged" for r in batch), "matches_sequential": same}
+    log(f"[bench] batch {B} RHS at {N}^3: matvec {ms:.2f} ms, solve {t_batch:.3f} s batched vs {t_seq:.3f} s "
+        f"sequential, its {min(its)}..{max(its)}")
+    return out
+
+
 def cpu_baseline(N, reps, k0):
     """Oracle (NumPy, complex64 as the reference's SINGLE path) matvec at the
     same size; the spectrum is a random tensor of the reference's dense
@@ -155,6 +195,9 @@ def main():
     ap.add_argument("--k0", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-n", type=int, default=96)
+    ap.add_argument("--batch-dirs", type=int, default=64)
     ap.add_argument("--k0s", type=float, nargs="*", default=[4.0, 8.0, 12.0],
                     help="wave numbers of the time-to-rtol solves")
     ap.add_argument("--copy-probe", action="store_true",
@@ -227,6 +270,10 @@ def main():
     if not args.no_solve:
         solve = solve_section(em, N, args.k0s)
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_section(em, args.batch_n, args.batch_dirs, 8.0)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, numpy)")
@@ -258,6 +305,7 @@ def main():
                      "traffic": measured_traffic(PASS_NAMES[dom], N, n_chi, es),
                      "algorithmic_bytes": pb[dom]},
         "solve": solve,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

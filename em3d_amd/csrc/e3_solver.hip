@@ -1,14 +1,16 @@
-// Device-resident BiCGStab on the FFT operator of a plan.
+// Device-resident BiCGStab on the FFT operator of a plan, for every
+// right-hand side of the plan's batch at once.
 //
 // Same recurrence, early exit on ||s||, statuses and residual-history
 // semantics as the reference solver (src/em3d/solvers/bicgstab.py:12-211),
-// with every scalar of the iteration kept on the device: the reductions end
-// in one-block "decide" kernels that compute rho, alpha, omega, the relative
-// residuals and the status, and append to the residual history.  Once a
-// status is decided, a device flag turns every later kernel of the solve
-// (vector updates, reductions and the five FFT passes of the matvec) into a
-// no-op, so the host enqueues iterations in chunks and reads the state back
-// once per chunk instead of six times per iteration.
+// run independently per right-hand side, with every scalar of the iteration
+// kept on the device: the reductions end in "decide" kernels (one block per
+// right-hand side) that compute rho, alpha, omega, the relative residuals and
+// the status, and append to that right-hand side's residual history.  Once a
+// right-hand side has a status its vector kernels return at once; once every
+// right-hand side has one (or, for t = A s, has stopped at ||s||), group flags
+// turn the five FFT passes of the batched matvec into no-ops too.  The host
+// enqueues iterations in chunks and reads the states back once per chunk.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,6 +58,7 @@ struct BiState {
   int skip_t;      // done || halt: the t matvec and its reductions are skipped
   int nhist;
   int hist_cap;
+  double last_rel;  // most recent history entry
 };
 
 __device__ __forceinline__ double wsum(double v) {
@@ -84,19 +87,6 @@ __device__ __forceinline__ void bsum(double* v) {
   }
 }
 
-template <int NV>
-__device__ __forceinline__ void final_sum(const double* part, int np, double* out) {
-  double v[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = 0.0;
-  for (int b = threadIdx.x; b < np; b += blockDim.x)
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] += part[b * NV + i];
-  bsum<NV>(v);
-#pragma unroll
-  for (int i = 0; i < NV; ++i) out[i] = v[i];
-}
-
 __device__ __forceinline__ double cabs_d(cpx<double> z) { return sqrt(z.re * z.re + z.im * z.im); }
 __device__ __forceinline__ cpx<double> cdiv(cpx<double> a, cpx<double> b) {
   const double d = b.re * b.re + b.im * b.im;
@@ -105,13 +95,6 @@ __device__ __forceinline__ cpx<double> cdiv(cpx<double> a, cpx<double> b) {
 template <typename T>
 __device__ __forceinline__ cpx<T> cast(cpx<double> z) { return mk((T)z.re, (T)z.im); }
 
-__device__ __forceinline__ void record(BiState* st, double* hist, long long* acts, double v) {
-  if (st->nhist < st->hist_cap) {
-    hist[st->nhist] = v;
-    acts[st->nhist] = st->mv;
-  }
-  st->nhist += 1;
-}
 __device__ __forceinline__ void finish(BiState* st, int status) {
   st->status = status;
   st->done = 1;
@@ -123,45 +106,57 @@ __device__ __forceinline__ void halt(BiState* st, int status) {
   st->skip_t = 1;
 }
 
-// ---- vector kernels (grid-stride, FP64 partial sums) ----------------------
-// part[b] = sum conj(x) y over block b's slice
+// ---- vector kernels: grid (blocks, batch); blockIdx.y selects the RHS ------
+// part[b][blk] = sum conj(x) y over the block's slice; skip_which 0: done, 1: skip_t
 template <typename T>
 __global__ __launch_bounds__(kRT) void k_dot(long long n, const cpx<T>* x, const cpx<T>* y, double* part,
-                                             const int* skip) {
-  if (*skip) return;
+                                             const BiState* st, int skip_which) {
+  const int b = blockIdx.y;
+  if (st && (skip_which ? st[b].skip_t : st[b].done)) return;
+  x += (long long)b * n; y += (long long)b * n;
   double v[2] = {0.0, 0.0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const cpx<T> a = x[i], b = y[i];
-    v[0] += (double)a.re * b.re + (double)a.im * b.im;
-    v[1] += (double)a.re * b.im - (double)a.im * b.re;
+    const cpx<T> a = x[i], c = y[i];
+    v[0] += (double)a.re * c.re + (double)a.im * c.im;
+    v[1] += (double)a.re * c.im - (double)a.im * c.re;
   }
   bsum<2>(v);
-  if (threadIdx.x == 0) { part[2 * blockIdx.x] = v[0]; part[2 * blockIdx.x + 1] = v[1]; }
+  if (threadIdx.x == 0) {
+    double* o = part + ((size_t)b * kRB + blockIdx.x) * 4;
+    o[0] = v[0]; o[1] = v[1];
+  }
 }
 
-// part[b] = (sum conj(t) t, sum conj(t) s)  (t t is complex like the reference's vdot)
+// part = (sum conj(t) t, sum conj(t) s)  (t t is complex like the reference's vdot)
 template <typename T>
 __global__ __launch_bounds__(kRT) void k_dot2(long long n, const cpx<T>* t, const cpx<T>* s, double* part,
-                                              const int* skip) {
-  if (*skip) return;
+                                              const BiState* st) {
+  const int b = blockIdx.y;
+  if (st[b].skip_t) return;
+  t += (long long)b * n; s += (long long)b * n;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const cpx<T> a = t[i], b = s[i];
+    const cpx<T> a = t[i], c = s[i];
     v[0] += (double)a.re * a.re + (double)a.im * a.im;
-    v[2] += (double)a.re * b.re + (double)a.im * b.im;
-    v[3] += (double)a.re * b.im - (double)a.im * b.re;
+    v[2] += (double)a.re * c.re + (double)a.im * c.im;
+    v[3] += (double)a.re * c.im - (double)a.im * c.re;
   }
   bsum<4>(v);
-  if (threadIdx.x == 0)
-    for (int i = 0; i < 4; ++i) part[4 * blockIdx.x + i] = v[i];
+  if (threadIdx.x == 0) {
+    double* o = part + ((size_t)b * kRB + blockIdx.x) * 4;
+    for (int i = 0; i < 4; ++i) o[i] = v[i];
+  }
 }
 
 // p = r (first iteration) or p = r + beta (p - omega v)
 template <typename T>
 __global__ __launch_bounds__(kRT) void k_p(long long n, cpx<T>* p, const cpx<T>* r, const cpx<T>* v, const BiState* st) {
-  if (st->done) return;
-  const bool first = st->it == 0;
-  const cpx<T> be = cast<T>(st->beta), om = cast<T>(st->omega);
+  const int b = blockIdx.y;
+  if (st[b].done) return;
+  const long long off = (long long)b * n;
+  p += off; r += off; v += off;
+  const bool first = st[b].it == 0;
+  const cpx<T> be = cast<T>(st[b].beta), om = cast<T>(st[b].omega);
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     if (first) {
       p[i] = r[i];
@@ -176,8 +171,11 @@ __global__ __launch_bounds__(kRT) void k_p(long long n, cpx<T>* p, const cpx<T>*
 template <typename T>
 __global__ __launch_bounds__(kRT) void k_s(long long n, cpx<T>* s, const cpx<T>* r, const cpx<T>* v, const BiState* st,
                                            double* part) {
-  if (st->done) return;
-  const cpx<T> al = cast<T>(st->alpha);
+  const int b = blockIdx.y;
+  if (st[b].done) return;
+  const long long off = (long long)b * n;
+  s += off; r += off; v += off;
+  const cpx<T> al = cast<T>(st[b].alpha);
   double acc = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const cpx<T> x = r[i] - al * v[i];
@@ -185,16 +183,19 @@ __global__ __launch_bounds__(kRT) void k_s(long long n, cpx<T>* s, const cpx<T>*
     acc += (double)x.re * x.re + (double)x.im * x.im;
   }
   bsum<1>(&acc);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+  if (threadIdx.x == 0) part[((size_t)b * kRB + blockIdx.x) * 4] = acc;
 }
 
 // u_alpha = u + alpha p; then (no halt) u = u_alpha + omega s, r = s - omega t, part = sum |r|^2
 template <typename T>
 __global__ __launch_bounds__(kRT) void k_ur(long long n, cpx<T>* u, const cpx<T>* p, const cpx<T>* s, cpx<T>* r,
                                             const cpx<T>* t, const BiState* st, double* part) {
-  if (st->done) return;
-  const bool h = st->halt != 0;
-  const cpx<T> al = cast<T>(st->alpha), om = cast<T>(st->omega);
+  const int b = blockIdx.y;
+  if (st[b].done) return;
+  const long long off = (long long)b * n;
+  u += off; p += off; s += off; r += off; t += off;
+  const bool h = st[b].halt != 0;
+  const cpx<T> al = cast<T>(st[b].alpha), om = cast<T>(st[b].omega);
   double acc = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const cpx<T> ua = u[i] + al * p[i];
@@ -209,14 +210,45 @@ __global__ __launch_bounds__(kRT) void k_ur(long long n, cpx<T>* u, const cpx<T>
     }
   }
   bsum<1>(&acc);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+  if (threadIdx.x == 0) part[((size_t)b * kRB + blockIdx.x) * 4] = acc;
 }
 
-// ---- decide kernels (one block) ---------------------------------------------
-__global__ __launch_bounds__(kRT) void k_rho(const double* part, int np, BiState* st) {
+// ---- decide kernels: one block per RHS ----------------------------------------
+struct Hist {
+  double* h;
+  long long* a;
+  int cap;
+};
+
+__device__ __forceinline__ const double* part_of(const double* part) { return part + (size_t)blockIdx.x * kRB * 4; }
+
+template <int NV>
+__device__ __forceinline__ void final_sum4(const double* part, int np, double* out) {
+  double v[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = 0.0;
+  for (int k = threadIdx.x; k < np; k += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] += part[(size_t)k * 4 + i];
+  bsum<NV>(v);
+#pragma unroll
+  for (int i = 0; i < NV; ++i) out[i] = v[i];
+}
+
+__device__ __forceinline__ void record_b(BiState* st, Hist H, double v) {
+  st->last_rel = v;
+  if (st->nhist < st->hist_cap) {
+    H.h[(size_t)blockIdx.x * H.cap + st->nhist] = v;
+    H.a[(size_t)blockIdx.x * H.cap + st->nhist] = st->mv;
+  }
+  st->nhist += 1;
+}
+
+__global__ __launch_bounds__(kRT) void k_rho(const double* part, int np, BiState* sts) {
+  BiState* st = sts + blockIdx.x;
   if (st->done) return;
   double v[2];
-  final_sum<2>(part, np, v);
+  final_sum4<2>(part_of(part), np, v);
   if (threadIdx.x) return;
   const cpx<double> rho = mk(v[0], v[1]);
   st->rho = rho;
@@ -228,10 +260,11 @@ __global__ __launch_bounds__(kRT) void k_rho(const double* part, int np, BiState
   if (st->use_max_act && st->mv + 1 > st->max_act) return finish(st, ST_MAX_ACTIONS);
 }
 
-__global__ __launch_bounds__(kRT) void k_alpha(const double* part, int np, BiState* st) {
+__global__ __launch_bounds__(kRT) void k_alpha(const double* part, int np, BiState* sts) {
+  BiState* st = sts + blockIdx.x;
   if (st->done) return;
   double v[2];
-  final_sum<2>(part, np, v);
+  final_sum4<2>(part_of(part), np, v);
   if (threadIdx.x) return;
   st->mv += 1;
   const cpx<double> den = mk(v[0], v[1]);
@@ -239,10 +272,11 @@ __global__ __launch_bounds__(kRT) void k_alpha(const double* part, int np, BiSta
   st->alpha = cdiv(st->rho, den);
 }
 
-__global__ __launch_bounds__(kRT) void k_sfin(const double* part, int np, BiState* st, double* hist, long long* acts) {
+__global__ __launch_bounds__(kRT) void k_sfin(const double* part, int np, BiState* sts, Hist H) {
+  BiState* st = sts + blockIdx.x;
   if (st->done) return;
   double v;
-  final_sum<1>(part, np, &v);
+  final_sum4<1>(part_of(part), np, &v);
   if (threadIdx.x) return;
   const double sr = sqrt(v) / st->rhs_norm;
   st->s_rel = sr;
@@ -252,33 +286,35 @@ __global__ __launch_bounds__(kRT) void k_sfin(const double* part, int np, BiStat
   else if (sr < st->tol) stop = ST_CONVERGED;
   else if (st->use_max_act && st->mv + 1 > st->max_act) stop = ST_MAX_ACTIONS;
   if (stop != ST_RUNNING) {
-    record(st, hist, acts, sr);
+    record_b(st, H, sr);
     halt(st, stop);
   }
 }
 
-__global__ __launch_bounds__(kRT) void k_omega(const double* part, int np, BiState* st, double* hist, long long* acts) {
+__global__ __launch_bounds__(kRT) void k_omega(const double* part, int np, BiState* sts, Hist H) {
+  BiState* st = sts + blockIdx.x;
   if (st->skip_t) return;
   double v[4];
-  final_sum<4>(part, np, v);
+  final_sum4<4>(part_of(part), np, v);
   if (threadIdx.x) return;
   st->mv += 1;
   const cpx<double> tt = mk(v[0], v[1]);
   if (cabs_d(tt) < 1e-30 * st->rhs_norm * st->rhs_norm) {
-    record(st, hist, acts, st->s_rel);
+    record_b(st, H, st->s_rel);
     return halt(st, ST_BREAKDOWN_T);
   }
   st->omega = cdiv(mk(v[2], v[3]), tt);
   if (cabs_d(st->omega) == 0.0) {
-    record(st, hist, acts, st->s_rel);
+    record_b(st, H, st->s_rel);
     return halt(st, ST_BREAKDOWN_OMEGA);
   }
 }
 
-__global__ __launch_bounds__(kRT) void k_rfin(const double* part, int np, BiState* st, double* hist, long long* acts) {
+__global__ __launch_bounds__(kRT) void k_rfin(const double* part, int np, BiState* sts, Hist H) {
+  BiState* st = sts + blockIdx.x;
   if (st->done) return;
   double v;
-  final_sum<1>(part, np, &v);
+  final_sum4<1>(part_of(part), np, &v);
   if (threadIdx.x) return;
   st->it += 1;
   if (st->halt) {
@@ -286,12 +322,43 @@ __global__ __launch_bounds__(kRT) void k_rfin(const double* part, int np, BiStat
     return;
   }
   const double rel = sqrt(v) / st->rhs_norm;
-  record(st, hist, acts, rel);
+  record_b(st, H, rel);
   if (!isfinite(rel)) return finish(st, ST_NONFINITE);
   if (rel < st->tol) return finish(st, ST_CONVERGED);
   if (st->use_divg && rel > st->divg) return finish(st, ST_DIVERGENCE);
   st->rho_prev = st->rho;
   if (st->it >= st->max_iter) return finish(st, ST_MAX_ITER);
+}
+
+// group flags of the batched matvecs: all_done = AND done, all_skip_t = AND skip_t
+struct Group {
+  int all_done;
+  int all_skip_t;
+};
+
+__global__ __launch_bounds__(kRT) void k_group(const BiState* sts, int nb, Group* g) {
+  int d = 1, s = 1;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    d &= sts[b].done ? 1 : 0;
+    s &= sts[b].skip_t ? 1 : 0;
+  }
+  __shared__ int sd[kRT], ss[kRT];
+  sd[threadIdx.x] = d;
+  ss[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)blockDim.x; ++k) { d &= sd[k]; s &= ss[k]; }
+    g->all_done = d;
+    g->all_skip_t = s;
+  }
+}
+
+// first history entry of every RHS (relative residual of u = 0 and its action count)
+__global__ void k_hist0(const BiState* sts, const double* rel0, int nb, Hist H) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  H.h[(size_t)b * H.cap] = rel0[b];
+  H.a[(size_t)b * H.cap] = sts[b].mv;
 }
 
 int sfail(const std::string& m) { return set_error(m); }
@@ -301,14 +368,14 @@ int chk(hipError_t e, const char* what) {
   return sfail(std::string("em3d_bicgstab: ") + what + ": " + hipGetErrorString(e));
 }
 
-size_t vec_bytes(const em3d_plan_s* plan) {
+long long field_elems(const em3d_plan_s* plan) {
   const Plan& p = plan->p;
-  return (size_t)p.batch * p.C * p.N[0] * p.N[1] * p.N[2] * elem_bytes(p.prec);
+  return (long long)p.C * p.N[0] * p.N[1] * p.N[2];
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// solver scratch: r, r_hat, p, v, s, t | partials | state
+// solver scratch: r, r_hat, p, v, s, t (batch fields each) | partials | states | group | rel0
 struct Work {
   void* r;
   void* rh;
@@ -318,29 +385,63 @@ struct Work {
   void* t;
   double* part;
   BiState* st;
+  Group* g;
+  double* rel0;
 };
 
-Work carve(void* base, size_t vb) {
+size_t part_bytes(int B) { return align256((size_t)B * kRB * 4 * sizeof(double)); }
+
+Work carve(void* base, size_t vb, int B) {
   Work w;
   char* c = (char*)base;
   const size_t a = align256(vb);
   w.r = c; w.rh = c + a; w.p = c + 2 * a; w.v = c + 3 * a; w.s = c + 4 * a; w.t = c + 5 * a;
-  w.part = (double*)(c + 6 * a);
-  w.st = (BiState*)(c + 6 * a + align256((size_t)kRB * 4 * sizeof(double)));
+  c += 6 * a;
+  w.part = (double*)c;
+  c += part_bytes(B);
+  w.st = (BiState*)c;
+  c += align256((size_t)B * sizeof(BiState));
+  w.g = (Group*)c;
+  c += align256(sizeof(Group));
+  w.rel0 = (double*)c;
   return w;
 }
 
+size_t work_total(size_t vb, int B) {
+  return 6 * align256(vb) + part_bytes(B) + align256((size_t)B * sizeof(BiState)) + align256(sizeof(Group)) +
+         align256((size_t)B * sizeof(double));
+}
+
 template <typename T>
-struct Launch {
-  long long n;
-  int nb;
-  hipStream_t st;
-  hipError_t err = hipSuccess;
-  void note() {
-    hipError_t e = hipGetLastError();
-    if (err == hipSuccess) err = e;
+void enqueue_iteration(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, void* u, void* work,
+                       const Work& w, long long n, int nb, int B, Hist H, hipStream_t stream, int* fail) {
+  const dim3 gv(nb, B), gd(B);
+  const int* all_done = &w.g->all_done;
+  const int* all_skip_t = &w.g->all_skip_t;
+  hipLaunchKernelGGL(k_dot<T>, gv, dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.r, w.part, w.st, 0);
+  hipLaunchKernelGGL(k_rho, gd, dim3(kRT), 0, stream, w.part, nb, w.st);
+  hipLaunchKernelGGL(k_p<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)w.p, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st);
+  hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
+  if (apply_plan(plan, khat, chi, chi_kind, w.p, w.v, work, stream, false, "em3d_bicgstab matvec", all_done)) {
+    *fail = 1;
+    return;
   }
-};
+  hipLaunchKernelGGL(k_dot<T>, gv, dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.v, w.part, w.st, 0);
+  hipLaunchKernelGGL(k_alpha, gd, dim3(kRT), 0, stream, w.part, nb, w.st);
+  hipLaunchKernelGGL(k_s<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)w.s, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st,
+                     w.part);
+  hipLaunchKernelGGL(k_sfin, gd, dim3(kRT), 0, stream, w.part, nb, w.st, H);
+  hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
+  if (apply_plan(plan, khat, chi, chi_kind, w.s, w.t, work, stream, false, "em3d_bicgstab matvec", all_skip_t)) {
+    *fail = 1;
+    return;
+  }
+  hipLaunchKernelGGL(k_dot2<T>, gv, dim3(kRT), 0, stream, n, (const cpx<T>*)w.t, (const cpx<T>*)w.s, w.part, w.st);
+  hipLaunchKernelGGL(k_omega, gd, dim3(kRT), 0, stream, w.part, nb, w.st, H);
+  hipLaunchKernelGGL(k_ur<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)u, (const cpx<T>*)w.p, (const cpx<T>*)w.s,
+                     (cpx<T>*)w.r, (const cpx<T>*)w.t, w.st, w.part);
+  hipLaunchKernelGGL(k_rfin, gd, dim3(kRT), 0, stream, w.part, nb, w.st, H);
+}
 
 }  // namespace
 
@@ -348,7 +449,8 @@ extern "C" {
 
 size_t em3d_bicgstab_work_bytes(em3d_plan_t plan) {
   if (!plan) return 0;
-  return 6 * align256(vec_bytes(plan)) + align256((size_t)kRB * 4 * sizeof(double)) + align256(sizeof(BiState));
+  const int B = plan->p.batch;
+  return work_total((size_t)B * field_elems(plan) * elem_bytes(plan->p.prec), B);
 }
 
 int em3d_bicgstab(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* rhs, void* u,
@@ -356,152 +458,118 @@ int em3d_bicgstab(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
                   long long* action_counts, int history_cap, em3d_solver_result* res, hipStream_t stream) {
   if (!plan || !khat || !rhs || !u || !work || !solver_work || !prm || !res || !history || !action_counts)
     return sfail("em3d_bicgstab: null argument");
-  if (plan->p.batch != 1) return sfail("em3d_bicgstab: the plan must have batch 1");
   if (history_cap < 1) return sfail("em3d_bicgstab: history_cap must be >= 1");
   const Plan& P = plan->p;
   const int prec = P.prec;
-  const long long n = (long long)P.C * P.N[0] * P.N[1] * P.N[2];
-  const size_t vb = vec_bytes(plan);
-  Work w = carve(solver_work, vb);
+  const int B = P.batch;
+  const long long n = field_elems(plan);
+  const size_t es = elem_bytes(prec);
+  const size_t vb = (size_t)B * n * es;
+  Work w = carve(solver_work, vb, B);
   const int nb = (int)std::min<long long>(kRB, (n + kRT - 1) / kRT);
+  const Hist H{history, action_counts, history_cap};
   hipError_t e;
-  *res = em3d_solver_result{};
+  for (int b = 0; b < B; ++b) res[b] = em3d_solver_result{};
 
-  // host-side start: ||f||, r = f - A 0 (= f: A 0 is exactly 0, counted as one action)
-  BiState hs{};
-  // the state's done flag must read 0 for the first reduction: clear the state first
-  if ((e = hipMemsetAsync(w.st, 0, sizeof(BiState), stream)) != hipSuccess) return chk(e, "hipMemsetAsync");
+  // ||f_b|| for every RHS (the states are not yet set: no skip)
   if (prec == PREC_DOUBLE)
-    hipLaunchKernelGGL(k_dot<double>, dim3(nb), dim3(kRT), 0, stream, n, (const cpx<double>*)rhs,
-                       (const cpx<double>*)rhs, w.part, (const int*)&w.st->done);
+    hipLaunchKernelGGL(k_dot<double>, dim3(nb, B), dim3(kRT), 0, stream, n, (const cpx<double>*)rhs,
+                       (const cpx<double>*)rhs, w.part, (const BiState*)nullptr, 0);
   else
-    hipLaunchKernelGGL(k_dot<float>, dim3(nb), dim3(kRT), 0, stream, n, (const cpx<float>*)rhs,
-                       (const cpx<float>*)rhs, w.part, (const int*)&w.st->done);
+    hipLaunchKernelGGL(k_dot<float>, dim3(nb, B), dim3(kRT), 0, stream, n, (const cpx<float>*)rhs,
+                       (const cpx<float>*)rhs, w.part, (const BiState*)nullptr, 0);
   if ((e = hipGetLastError()) != hipSuccess) return chk(e, "norm launch");
-  std::vector<double> hp((size_t)nb * 2);
+  std::vector<double> hp((size_t)B * kRB * 4);
   if ((e = hipMemcpyAsync(hp.data(), w.part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
       (e = hipStreamSynchronize(stream)) != hipSuccess)
     return chk(e, "norm readback");
-  double nn = 0.0;
-  for (int b = 0; b < nb; ++b) nn += hp[2 * b];
-  const double rhs_norm = std::sqrt(nn);
   if ((e = hipMemsetAsync(u, 0, vb, stream)) != hipSuccess) return chk(e, "hipMemsetAsync(u)");
-  // first history entry (device arrays): ||f - A 0|| / ||f|| and its action count
-  static thread_local double h0;
-  static thread_local long long a0;
-  if (rhs_norm == 0.0) {
-    h0 = 0.0;
-    a0 = 0;
-    res->status = ST_CONVERGED;
-    res->history_len = 1;
-    if ((e = hipMemcpyAsync(history, &h0, sizeof(double), hipMemcpyHostToDevice, stream)) != hipSuccess ||
-        (e = hipMemcpyAsync(action_counts, &a0, sizeof(long long), hipMemcpyHostToDevice, stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(stream)) != hipSuccess)
-      return chk(e, "history");
-    return 0;
-  }
-  const double tol = std::max(prm->rtol, prm->atol / rhs_norm);
-  const double rel0 = rhs_norm / rhs_norm;   // ||f - A 0|| / ||f||
-  long long mv = 1;
-  h0 = rel0;
-  a0 = mv;
-  if ((e = hipMemcpyAsync(history, &h0, sizeof(double), hipMemcpyHostToDevice, stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(action_counts, &a0, sizeof(long long), hipMemcpyHostToDevice, stream)) != hipSuccess)
-    return chk(e, "history");
-  res->history_len = 1;
-  res->matvec_count = (int)mv;
-  res->true_final_residual = rel0;
-  if (!std::isfinite(rel0)) {
-    res->status = ST_NONFINITE;
-    return chk(hipStreamSynchronize(stream), "sync");
-  }
-  if (rel0 < tol) {
-    res->status = ST_CONVERGED;
-    return chk(hipStreamSynchronize(stream), "sync");
-  }
-  if (prm->max_operator_actions >= 0 && mv >= prm->max_operator_actions) {
-    res->status = ST_MAX_ACTIONS;
-    return chk(hipStreamSynchronize(stream), "sync");
-  }
-  if (prm->max_iter <= 0) {
-    res->status = ST_MAX_ITER;
-    return chk(hipStreamSynchronize(stream), "sync");
-  }
 
-  // r = r_hat = f, p = v = 0, state
+  // per-RHS start: r = f - A 0 = f (A 0 is exactly 0, counted as one action)
+  std::vector<BiState> hs((size_t)B);
+  std::vector<double> rel0((size_t)B);
+  bool any_running = false;
+  for (int b = 0; b < B; ++b) {
+    double nn = 0.0;
+    for (int k = 0; k < nb; ++k) nn += hp[((size_t)b * kRB + k) * 4];
+    const double rn = std::sqrt(nn);
+    BiState& s = hs[b];
+    s = BiState{};
+    s.rhs_norm = rn;
+    s.divg = prm->divergence_guard;
+    s.use_divg = prm->divergence_guard >= 0.0 ? 1 : 0;
+    s.max_act = prm->max_operator_actions;
+    s.use_max_act = prm->max_operator_actions >= 0 ? 1 : 0;
+    s.max_iter = prm->max_iter;
+    s.rho_prev = mk(1.0, 0.0);
+    s.alpha = mk(1.0, 0.0);
+    s.omega = mk(1.0, 0.0);
+    s.nhist = 1;
+    s.hist_cap = history_cap;
+    s.status = ST_RUNNING;
+    if (rn == 0.0) {
+      rel0[b] = 0.0;
+      s.mv = 0;
+      s.status = ST_CONVERGED;
+    } else {
+      s.tol = std::max(prm->rtol, prm->atol / rn);
+      rel0[b] = rn / rn;
+      s.mv = 1;
+      if (!std::isfinite(rel0[b])) s.status = ST_NONFINITE;
+      else if (rel0[b] < s.tol) s.status = ST_CONVERGED;
+      else if (prm->max_operator_actions >= 0 && s.mv >= prm->max_operator_actions) s.status = ST_MAX_ACTIONS;
+      else if (prm->max_iter <= 0) s.status = ST_MAX_ITER;
+    }
+    s.last_rel = rel0[b];
+    if (s.status != ST_RUNNING) {
+      s.done = 1;
+      s.skip_t = 1;
+    } else {
+      any_running = true;
+    }
+  }
+  // r = r_hat = f, p = v = 0, states, first history entries
   if ((e = hipMemcpyAsync(w.r, rhs, vb, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
       (e = hipMemcpyAsync(w.rh, rhs, vb, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
-      (e = hipMemsetAsync(w.p, 0, vb, stream)) != hipSuccess || (e = hipMemsetAsync(w.v, 0, vb, stream)) != hipSuccess)
-    return chk(e, "init vectors");
-  hs.rhs_norm = rhs_norm;
-  hs.tol = tol;
-  hs.divg = prm->divergence_guard;
-  hs.use_divg = prm->divergence_guard >= 0.0 ? 1 : 0;
-  hs.max_act = prm->max_operator_actions;
-  hs.use_max_act = prm->max_operator_actions >= 0 ? 1 : 0;
-  hs.max_iter = prm->max_iter;
-  hs.rho_prev = mk(1.0, 0.0);
-  hs.alpha = mk(1.0, 0.0);
-  hs.omega = mk(1.0, 0.0);
-  hs.mv = mv;
-  hs.status = ST_RUNNING;
-  hs.nhist = 1;
-  hs.hist_cap = history_cap;
-  if ((e = hipMemcpyAsync(w.st, &hs, sizeof(BiState), hipMemcpyHostToDevice, stream)) != hipSuccess)
-    return chk(e, "state upload");
+      (e = hipMemsetAsync(w.p, 0, vb, stream)) != hipSuccess || (e = hipMemsetAsync(w.v, 0, vb, stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(w.st, hs.data(), (size_t)B * sizeof(BiState), hipMemcpyHostToDevice, stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(w.rel0, rel0.data(), (size_t)B * sizeof(double), hipMemcpyHostToDevice, stream)) != hipSuccess)
+    return chk(e, "init");
+  hipLaunchKernelGGL(k_hist0, dim3((B + 63) / 64), dim3(64), 0, stream, w.st, w.rel0, B, H);
+  if ((e = hipGetLastError()) != hipSuccess) return chk(e, "history launch");
 
   const int chunk = prm->chunk > 0 ? prm->chunk : 4;
-  const int* done = &w.st->done;
-  const int* skip_t = &w.st->skip_t;
-  static thread_local BiState cur;
-  cur = BiState{};
-  for (int it0 = 0; it0 < prm->max_iter; it0 += chunk) {
-    for (int k = 0; k < chunk && it0 + k < prm->max_iter; ++k) {
-      if (prec == PREC_DOUBLE) {
-        using T = double;
-        hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.r, w.part, done);
-        hipLaunchKernelGGL(k_rho, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st);
-        hipLaunchKernelGGL(k_p<T>, dim3(nb), dim3(kRT), 0, stream, n, (cpx<T>*)w.p, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st);
-        if (apply_plan(plan, khat, chi, chi_kind, w.p, w.v, work, stream, false, "em3d_bicgstab matvec", done)) return 1;
-        hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.v, w.part, done);
-        hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st);
-        hipLaunchKernelGGL(k_s<T>, dim3(nb), dim3(kRT), 0, stream, n, (cpx<T>*)w.s, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st, w.part);
-        hipLaunchKernelGGL(k_sfin, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st, history, action_counts);
-        if (apply_plan(plan, khat, chi, chi_kind, w.s, w.t, work, stream, false, "em3d_bicgstab matvec", skip_t)) return 1;
-        hipLaunchKernelGGL(k_dot2<T>, dim3(nb), dim3(kRT), 0, stream, n, (const cpx<T>*)w.t, (const cpx<T>*)w.s, w.part, skip_t);
-        hipLaunchKernelGGL(k_omega, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st, history, action_counts);
-        hipLaunchKernelGGL(k_ur<T>, dim3(nb), dim3(kRT), 0, stream, n, (cpx<T>*)u, (const cpx<T>*)w.p, (const cpx<T>*)w.s,
-                           (cpx<T>*)w.r, (const cpx<T>*)w.t, w.st, w.part);
-        hipLaunchKernelGGL(k_rfin, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st, history, action_counts);
-      } else {
-        using T = float;
-        hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.r, w.part, done);
-        hipLaunchKernelGGL(k_rho, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st);
-        hipLaunchKernelGGL(k_p<T>, dim3(nb), dim3(kRT), 0, stream, n, (cpx<T>*)w.p, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st);
-        if (apply_plan(plan, khat, chi, chi_kind, w.p, w.v, work, stream, false, "em3d_bicgstab matvec", done)) return 1;
-        hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.v, w.part, done);
-        hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st);
-        hipLaunchKernelGGL(k_s<T>, dim3(nb), dim3(kRT), 0, stream, n, (cpx<T>*)w.s, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st, w.part);
-        hipLaunchKernelGGL(k_sfin, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st, history, action_counts);
-        if (apply_plan(plan, khat, chi, chi_kind, w.s, w.t, work, stream, false, "em3d_bicgstab matvec", skip_t)) return 1;
-        hipLaunchKernelGGL(k_dot2<T>, dim3(nb), dim3(kRT), 0, stream, n, (const cpx<T>*)w.t, (const cpx<T>*)w.s, w.part, skip_t);
-        hipLaunchKernelGGL(k_omega, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st, history, action_counts);
-        hipLaunchKernelGGL(k_ur<T>, dim3(nb), dim3(kRT), 0, stream, n, (cpx<T>*)u, (const cpx<T>*)w.p, (const cpx<T>*)w.s,
-                           (cpx<T>*)w.r, (const cpx<T>*)w.t, w.st, w.part);
-        hipLaunchKernelGGL(k_rfin, dim3(1), dim3(kRT), 0, stream, w.part, nb, w.st, history, action_counts);
+  std::vector<BiState> cur(hs);
+  if (any_running) {
+    for (int it0 = 0; it0 < prm->max_iter; it0 += chunk) {
+      for (int k = 0; k < chunk && it0 + k < prm->max_iter; ++k) {
+        int fail = 0;
+        if (prec == PREC_DOUBLE)
+          enqueue_iteration<double>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, stream, &fail);
+        else
+          enqueue_iteration<float>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, stream, &fail);
+        if (fail) return 1;
+        if ((e = hipGetLastError()) != hipSuccess) return chk(e, "iteration launch");
       }
-      if ((e = hipGetLastError()) != hipSuccess) return chk(e, "iteration launch");
+      if ((e = hipMemcpyAsync(cur.data(), w.st, (size_t)B * sizeof(BiState), hipMemcpyDeviceToHost, stream)) !=
+              hipSuccess ||
+          (e = hipStreamSynchronize(stream)) != hipSuccess)
+        return chk(e, "state readback");
+      bool all = true;
+      for (int b = 0; b < B; ++b) all = all && cur[b].done;
+      if (all) break;
     }
-    if ((e = hipMemcpyAsync(&cur, w.st, sizeof(BiState), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(stream)) != hipSuccess)
-      return chk(e, "state readback");
-    if (cur.done) break;
+  } else if ((e = hipStreamSynchronize(stream)) != hipSuccess) {
+    return chk(e, "sync");
   }
-  if (!cur.done) cur.status = ST_MAX_ITER;   // k_rfin decides at max_iter; kept for safety
-  res->status = cur.status;
-  res->iterations = cur.it;
-  res->matvec_count = (int)cur.mv;
-  res->history_len = cur.nhist < history_cap ? cur.nhist : history_cap;
+  for (int b = 0; b < B; ++b) {
+    const BiState& s = cur[b];
+    res[b].status = s.done ? s.status : ST_MAX_ITER;   // k_rfin decides at max_iter; kept for safety
+    res[b].iterations = s.it;
+    res[b].matvec_count = (int)s.mv;
+    res[b].history_len = s.nhist < history_cap ? s.nhist : history_cap;
+    res[b].true_final_residual = s.last_rel;
+  }
   return 0;
 }
 

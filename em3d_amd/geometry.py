@@ -140,3 +140,45 @@ def sample_contrast_tensor(grid: Grid, *, background_eps_r, feature_eps_r, geome
     ident = grid.backend.array(np.eye(3, dtype=np.complex128)[:, :, None, None, None],
                                dtype=grid.backend.complex_dtype)
     return eps - ident
+
+
+def _eps_record(value) -> np.ndarray:
+    m = _material(value, "eps_r")
+    out = np.empty(18, dtype=np.float64)
+    out[0::2] = m.real.ravel()
+    out[1::2] = m.imag.ravel()
+    return out
+
+
+def ellipsoids_contrast(grid: Grid, inclusions, *, background_eps_r=1.0, kind: str = "full"):
+    """Device contrast ``chi = eps_r - I`` of a background holding ellipsoidal
+    inclusions ``[(Ellipsoid, eps_r), ...]`` (eps_r scalar or 3x3), tested at
+    cell centres exactly as ``center_mask`` (FP64, same tolerance), later
+    inclusions overwriting earlier ones; computed by one HIP launch.
+
+    ``kind="full"`` returns the (3, 3, Nx, Ny, Nz) tensor ``Problem`` takes;
+    "diag" and "iso" return the packed (3, Nx, Ny, Nz) / (Nx, Ny, Nz) forms the
+    kernels read (valid only when every eps_r is diagonal / scalar)."""
+    import torch
+
+    from . import _lib
+    be = grid.backend
+    codes = {"iso": 1, "diag": 2, "full": 3}
+    if kind not in codes:
+        raise ValueError(f"kind must be one of {sorted(codes)}, got {kind!r}")
+    recs = []
+    for geom, eps in inclusions:
+        if not isinstance(geom, Ellipsoid):
+            raise TypeError(f"inclusions must be Ellipsoid geometries, got {type(geom)!r}")
+        recs.append(np.concatenate([np.asarray(geom.center, np.float64), np.asarray(geom.radii, np.float64),
+                                    _eps_record(eps)]))
+    dev = be.torch_device
+    ell = torch.as_tensor(np.ascontiguousarray(np.array(recs, dtype=np.float64).reshape(-1, 24))).to(dev)
+    bg = torch.as_tensor(_eps_record(background_eps_r)).to(dev)
+    shape = {"iso": (), "diag": (3,), "full": (3, 3)}[kind] + tuple(grid.N)
+    chi = torch.empty(shape, dtype=be.precision.torch_complex_dtype, device=dev)
+    _lib.check(_lib.lib().em3d_ellipsoid_contrast(*grid.N, grid.x.data_ptr(), grid.y.data_ptr(), grid.z.data_ptr(),
+                                                  be.precision.code, len(recs), ell.data_ptr() if recs else None,
+                                                  bg.data_ptr(), _tolerance(grid), codes[kind], chi.data_ptr(),
+                                                  _lib.stream_handle(dev)), "em3d_ellipsoid_contrast")
+    return chi

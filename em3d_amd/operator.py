@@ -307,39 +307,46 @@ class _FieldApplier:
             raise ValueError(f"field shape {tuple(t.shape)} != expected {fs} (or (batch,) + {fs})")
         return t.contiguous(), batch
 
-    def _device_bicgstab(self, rhs, cfg, khat, chi_kind, chi):
-        """Device-resident BiCGStab of libem3d_hip on this operator (batch-1 field)."""
+    def _device_bicgstab(self, rhs, cfg, khat, chi_kind, chi, batched: bool = False):
+        """Device-resident BiCGStab of libem3d_hip on this operator.  ``batched``:
+        rhs carries a leading batch axis and a list of results (one per RHS) is
+        returned; every RHS runs its own recurrence and status."""
         import torch
 
         from .solvers.base import SolverResult
         t, batch = self._prepare_input(rhs)
-        if batch != 1:
-            raise ValueError("device BiCGStab solves one right-hand side")
+        if batched != (t.dim() == len(self._field_shape()) + 1):
+            raise ValueError("batched solves take (batch,) + field shape; single solves take the field shape")
         be = self._be
-        plan = HipPlan.get(self._N, self._ncomp, 1, be.precision, be.device_index)
+        plan = HipPlan.get(self._N, self._ncomp, batch, be.precision, be.device_index)
         lib = _lib.lib()
         sw = torch.empty(int(lib.em3d_bicgstab_work_bytes(plan.handle)), dtype=torch.uint8, device=be.torch_device)
         cap = int(cfg.max_iter) + 2
-        hist = torch.zeros(cap, dtype=torch.float64, device=be.torch_device)
-        acts = torch.zeros(cap, dtype=torch.int64, device=be.torch_device)
+        hist = torch.zeros((batch, cap), dtype=torch.float64, device=be.torch_device)
+        acts = torch.zeros((batch, cap), dtype=torch.int64, device=be.torch_device)
         u = torch.empty_like(t)
         prm = _lib.BiCGStabParams(int(cfg.max_iter), float(cfg.rtol), float(cfg.atol),
                                   -1.0 if cfg.divergence_guard is None else float(cfg.divergence_guard),
                                   -1 if cfg.max_operator_actions is None else int(cfg.max_operator_actions),
                                   int(getattr(cfg, "chunk", 4) or 4))
-        res = _lib.SolverResultC()
+        res = (_lib.SolverResultC * batch)()
         _lib.check(lib.em3d_bicgstab(plan.handle, khat.data_ptr(), chi.data_ptr() if chi is not None else None,
                                      chi_kind, t.data_ptr(), u.data_ptr(), self._workspace(plan).data_ptr(),
                                      sw.data_ptr(), C.byref(prm), hist.data_ptr(), acts.data_ptr(), cap,
-                                     C.byref(res), _lib.stream_handle(be.torch_device)), "em3d_bicgstab")
-        n = int(res.history_len)
-        history = [float(x) for x in hist[:n].cpu().tolist()]
-        counts = [int(x) for x in acts[:n].cpu().tolist()]
-        status = _lib.STATUS[int(res.status)]
-        return SolverResult(u=u, iterations=int(res.iterations), residual_history=history,
-                            converged=status == "converged", matvec_count=int(res.matvec_count), rmatvec_count=0,
-                            residual_action_counts=counts, status=status,
-                            true_final_residual=float(history[-1]) if history else float("inf"))
+                                     res, _lib.stream_handle(be.torch_device)), "em3d_bicgstab")
+        hist_h, acts_h = hist.cpu(), acts.cpu()
+        out = []
+        for b in range(batch):
+            n = int(res[b].history_len)
+            history = [float(x) for x in hist_h[b, :n].tolist()]
+            counts = [int(x) for x in acts_h[b, :n].tolist()]
+            status = _lib.STATUS[int(res[b].status)]
+            out.append(SolverResult(u=u[b] if batched else u, iterations=int(res[b].iterations),
+                                    residual_history=history, converged=status == "converged",
+                                    matvec_count=int(res[b].matvec_count), rmatvec_count=0,
+                                    residual_action_counts=counts, status=status,
+                                    true_final_residual=float(history[-1]) if history else float("inf")))
+        return out if batched else out[0]
 
     def _apply(self, field, khat, chi_kind, chi, adjoint: bool):
         import torch
@@ -374,8 +381,8 @@ class _SingleCompanion(_FieldApplier):
     def rmatvec(self, field):
         return self._apply(field, self._khat, self._chi_kind, self._chi, True)
 
-    def device_bicgstab(self, rhs, cfg):
-        return self._device_bicgstab(rhs, cfg, self._khat, self._chi_kind, self._chi)
+    def device_bicgstab(self, rhs, cfg, batched: bool = False):
+        return self._device_bicgstab(rhs, cfg, self._khat, self._chi_kind, self._chi, batched)
 
 
 class Operator(_FieldApplier):
@@ -416,9 +423,10 @@ class Operator(_FieldApplier):
     def matvec(self, field):
         return self._apply(field, self._prepared_kernel.khat_octant, self._chi_kind, self._chi, False)
 
-    def device_bicgstab(self, rhs, cfg):
+    def device_bicgstab(self, rhs, cfg, batched: bool = False):
         """BiCGStab with all iteration scalars on the device (libem3d_hip)."""
-        return self._device_bicgstab(rhs, cfg, self._prepared_kernel.khat_octant, self._chi_kind, self._chi)
+        return self._device_bicgstab(rhs, cfg, self._prepared_kernel.khat_octant, self._chi_kind, self._chi,
+                                     batched)
 
     def rmatvec(self, field):
         if not self._prepared_kernel.supports_adjoint:

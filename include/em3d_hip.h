@@ -145,6 +145,34 @@ int em3d_axpbypcz(int precision, long long n, double a_re, double a_im, const vo
 int em3d_convert_axpy(long long n, int src_precision, const void* src, double a_re, double a_im, int dst_precision,
                       void* dst, int accumulate, hipStream_t stream);
 
+/* Batch of plane waves f_b = A_b exp(i sign (k o_b . r + phi0_b)), out
+ * (batch, 3, nx, ny, nz) in `precision`; ax/ay/az are the grid's cell-centre
+ * axes on the device in the precision's real type; params is a DEVICE array of
+ * batch x 10 doubles: o(3), Re A(3), Im A(3), phi0.  The phase is evaluated in
+ * FP64.  Replaces one flat_wave_vec call per field (wave.py:9-37). */
+int em3d_plane_waves(int nx, int ny, int nz, const void* ax, const void* ay, const void* az, int precision, int batch,
+                     const double* params, double k, int sign, void* out, hipStream_t stream);
+
+/* Contrast chi = eps - I of a background with ellipsoidal inclusions tested
+ * at cell centres in FP64 with tolerance `tol` (center_mask and
+ * sample_contrast_tensor, geometry.py:62-93, 233-252); later ellipsoids
+ * overwrite earlier ones.  ellipsoids: DEVICE array of n x 24 doubles
+ * (centre(3), radii(3), eps 3x3 complex row-major re/im); background: DEVICE
+ * array of 18 doubles.  chi_kind selects the packed output: 1 ISO (eps[0][0]),
+ * 2 DIAG, 3 FULL. */
+int em3d_ellipsoid_contrast(int nx, int ny, int nz, const void* ax, const void* ay, const void* az, int precision,
+                            int n_ellipsoids, const double* ellipsoids, const double* background, double tol,
+                            int chi_kind, void* chi, hipStream_t stream);
+
+/* Acoustic eta of a background with ellipsoidal inclusions tested in the
+ * axes' real type without tolerance (eta_ellipsoid,
+ * acoustics/materials.py:62-83); later inclusions overwrite earlier ones.
+ * inclusions: DEVICE array of n x 8 doubles (centre(3), radii(3), Re eta,
+ * Im eta). */
+int em3d_eta_inclusions(int nx, int ny, int nz, const void* ax, const void* ay, const void* az, int precision,
+                        int n_inclusions, const double* inclusions, double outside_re, double outside_im, void* eta,
+                        hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

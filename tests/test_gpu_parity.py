@@ -268,3 +268,90 @@ def test_mixed_refinement_fallback_and_errors(em):
     be32 = _be(em, "single")
     with pytest.raises(ValueError, match="FP64"):
         em.MixedBiCGStab(em.SolverConfig()).solve(em.Operator(_config2(em, be32, N, k)), prob.wave)
+
+
+def test_batched_bicgstab_matches_single_solves(em):
+    """One device-resident run over a batch of RHS (incident directions x
+    polarisations, a zero RHS and a NaN RHS) gives each RHS exactly the
+    iterations, status, history and solution of its own single solve."""
+    import torch
+    from em3d_amd.sources import incident_set, plane_waves
+    N, k = (16, 16, 16), 6.0
+    _say("[batched bicgstab] 16^3 sphere eps 4, 6 RHS")
+    for prec in ("double", "single"):
+        be = _be(em, prec)
+        grid = em.Grid(N=N, L=(1, 1, 1), center=(0, 0, 0), backend=be)
+        chi = em.geometry.sample_contrast_tensor(grid, background_eps_r=1.0, feature_eps_r=4.0,
+                                                 geometry=em.geometry.Ellipsoid(center=(0, 0, 0), radii=(0.3,) * 3))
+        o, a = incident_set(2, seed=4)
+        waves = plane_waves(grid, k=k, orients=o, amplitudes=a)
+        zero = torch.zeros_like(waves[0])
+        bad = waves[1].clone()
+        bad[0, 1, 2, 3] = float("nan")
+        rhs = torch.stack([waves[0], zero, waves[1], bad, waves[2], waves[3]])
+        prob = em.Problem(grid=grid, eps_tensor=chi, wave=waves[0], k0=k, volume=1.0)
+        op = em.Operator(prob)
+        cfg = em.SolverConfig(max_iter=200, rtol=1e-8 if prec == "double" else 1e-5)
+        batch = em.BiCGStab(cfg).solve_batch(op, rhs)
+        assert [r.status for r in batch][1] == "converged" and batch[1].matvec_count == 0
+        assert batch[3].status == "nonfinite"
+        for b in range(rhs.shape[0]):
+            single = em.BiCGStab(cfg).solve(op, rhs[b].contiguous())
+            got = batch[b]
+            assert (got.status, got.iterations, got.matvec_count) == (single.status, single.iterations,
+                                                                        single.matvec_count), b
+            assert got.residual_action_counts == single.residual_action_counts
+            np.testing.assert_allclose(got.residual_history, single.residual_history, rtol=1e-12, atol=0)
+            if single.status != "nonfinite":
+                assert torch.equal(got.u, single.u), b
+
+
+@pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("single", 2e-6)])
+def test_plane_waves_match_flat_wave_vec(em, prec, tol):
+    from em3d_amd.sources import incident_set, plane_waves
+    be = _be(em, prec)
+    grid = em.Grid(N=(12, 10, 8), L=(1.0, 0.8, 1.3), center=(0.1, -0.2, 0.05), backend=be)
+    o, a = incident_set(3, seed=7)
+    a = a * (0.5 - 0.25j)
+    dev = plane_waves(grid, k=9.0, orients=o, amplitudes=a, phi0=0.3, sign=-1)
+    for b in range(o.shape[0]):
+        ref = em.flat_wave_vec(grid, k=9.0, orient=tuple(o[b]), amplitude=tuple(a[b]), phi0=0.3, sign=-1)
+        assert _rel(_host(dev[b]), _host(ref)) < tol
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_device_materials_match_host(em, prec):
+    """Device ellipsoid contrast and eta inclusions equal the host sampling
+    bit for bit, boundary cells included (a sphere through cell centres)."""
+    import torch
+    from em3d_amd.acoustics.materials import eta_ellipsoid, eta_inclusions, random_spheres
+    be = _be(em, prec)
+    grid = em.Grid(N=(16, 16, 16), L=(1, 1, 1), center=(0, 0, 0), backend=be)
+    geo = em.geometry
+    eps1 = np.diag([2.5 + 0.1j, 3.0 + 0.2j, 2.0])
+    eps2 = np.array([[2.0, 0.3j, 0], [-0.3j, 2.0, 0], [0, 0, 1.5]])
+    # radius 0.28125 = 4.5 cells: centres exactly on the surface test the tolerance
+    e1, e2 = geo.Ellipsoid(center=(0, 0, 0), radii=(0.28125,) * 3), geo.Ellipsoid(center=(0.1, 0, 0),
+                                                                               radii=(0.2, 0.15, 0.1))
+    ref1 = geo.sample_contrast_tensor(grid, background_eps_r=1.0, feature_eps_r=eps1, geometry=e1)
+    assert torch.equal(geo.ellipsoids_contrast(grid, [(e1, eps1)]), ref1)
+    d = geo.ellipsoids_contrast(grid, [(e1, eps1)], kind="diag")
+    assert torch.equal(d, torch.stack([ref1[i, i] for i in range(3)]))
+    two = geo.ellipsoids_contrast(grid, [(e1, eps1), (e2, eps2)], background_eps_r=1.2)
+    m1 = torch.as_tensor(geo.center_mask(e1, grid), device=two.device)
+    m2 = torch.as_tensor(geo.center_mask(e2, grid), device=two.device)
+    for a in range(3):
+        for c in range(3):
+            bg = (1.2 if a == c else 0.0) - (1.0 if a == c else 0.0)
+            want = torch.full(grid.N, bg, dtype=two.dtype, device=two.device)
+            want = torch.where(m1, torch.tensor(eps1[a, c] - (a == c), dtype=two.dtype, device=two.device), want)
+            want = torch.where(m2, torch.tensor(eps2[a, c] - (a == c), dtype=two.dtype, device=two.device), want)
+            assert torch.equal(two[a, c], want), (a, c)
+    spheres = random_spheres(8, seed=3, radius=(0.1, 0.2))
+    spheres.append(((0.0, 0.0, 0.0), (0.28125,) * 3, 2.0 + 0.1j))
+    eta = eta_inclusions(grid, spheres)
+    want = torch.ones(grid.N, dtype=eta.dtype, device=eta.device)
+    for c0, r, v in spheres:
+        m = eta_ellipsoid(grid, center=c0, radii=r, eta_inside=1.0, eta_outside=0.0) != 0
+        want = torch.where(m, torch.tensor(v, dtype=eta.dtype, device=eta.device), want)
+    assert torch.equal(eta, want)
