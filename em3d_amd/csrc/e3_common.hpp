@@ -40,6 +40,7 @@ template <typename T> E3_HD cpx<T> operator*(cpx<T> a, cpx<T> b) {
   return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 template <typename T> E3_HD cpx<T> operator*(T s, cpx<T> a) { return mk(s * a.re, s * a.im); }
+template <typename T> E3_HD cpx<T> operator*(cpx<T> a, T s) { return mk(a.re * s, a.im * s); }
 template <typename T> E3_HD cpx<T> conj(cpx<T> a) { return mk(a.re, -a.im); }
 template <typename T> E3_HD cpx<T> czero() { return mk(T(0), T(0)); }
 // conj(a) * b

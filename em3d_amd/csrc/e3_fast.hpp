@@ -55,10 +55,121 @@ constexpr int bit_reverse(int i) {
   return r;
 }
 
-// In-register DFT of power-of-two size R <= 32 (iterative radix-2, DIT).
+constexpr bool is_pow2(int r) { return r > 0 && (r & (r - 1)) == 0; }
+
+// cos and sin of 2 pi m / R, evaluated at compile time: the angle is reduced
+// exactly to the first quadrant with integer arithmetic, then summed as Taylor
+// series (|x| <= pi/2, 30 terms: truncation far below double rounding).
+constexpr double taylor_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int n = 1; n < 30; ++n) {
+    term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double taylor_sin(double x) {
+  double term = x, sum = x;
+  for (int n = 1; n < 30; ++n) {
+    term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double kHalfPi = 1.57079632679489661923;
+
+template <int R>
+struct Roots {
+  double c[R], s[R];
+  constexpr Roots() : c(), s() {
+    for (int m = 0; m < R; ++m) {
+      const int q = (4 * m) / R;                   // quadrant of 2 pi m / R
+      const double x = kHalfPi * (double)(4 * m - q * R) / (double)R;
+      const double cr = taylor_cos(x), sr = taylor_sin(x);
+      c[m] = q == 0 ? cr : q == 1 ? -sr : q == 2 ? -cr : sr;
+      s[m] = q == 0 ? sr : q == 1 ? cr : q == 2 ? -sr : -cr;
+    }
+  }
+};
+template <int R>
+inline constexpr Roots<R> kRoots{};
+
+// b * exp(-/+ 2 pi i e / R) (forward / inverse)
+template <typename T, int R, bool INV>
+E3_HD cpx<T> rotR(cpx<T> b, int e) {
+  if (e == 0) return b;
+  const T c = (T)kRoots<R>.c[e];
+  const T s = INV ? (T)kRoots<R>.s[e] : -(T)kRoots<R>.s[e];
+  return mk(b.re * c - b.im * s, b.re * s + b.im * c);
+}
+
+// In-place DFT of 3 or 5 points.
+template <typename T, int P, bool INV>
+E3_HD void dft_odd(cpx<T>* x) {
+  if constexpr (P == 3) {
+    const T h = (T)0.86602540378443864676;   // sin(2 pi / 3)
+    const cpx<T> t1 = x[1] + x[2], t2 = x[1] - x[2];
+    const cpx<T> r = x[0] - t1 * (T)0.5;
+    const cpx<T> i = mul_mi<T, INV>(t2 * h);
+    x[0] = x[0] + t1;
+    x[1] = r + i;
+    x[2] = r - i;
+  } else {
+    static_assert(P == 5, "odd radix must be 3 or 5");
+    const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;   // cos(2 pi / 5), cos(4 pi / 5)
+    const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;    // sin(2 pi / 5), sin(4 pi / 5)
+    const cpx<T> a1 = x[1] + x[4], b1 = x[1] - x[4], a2 = x[2] + x[3], b2 = x[2] - x[3];
+    const cpx<T> r1 = x[0] + a1 * c1 + a2 * c2, r2 = x[0] + a1 * c2 + a2 * c1;
+    const cpx<T> i1 = mul_mi<T, INV>(b1 * s1 + b2 * s2), i2 = mul_mi<T, INV>(b1 * s2 - b2 * s1);
+    x[0] = x[0] + a1 + a2;
+    x[1] = r1 + i1;
+    x[4] = r1 - i1;
+    x[2] = r2 + i2;
+    x[3] = r2 - i2;
+  }
+}
+
+template <typename T, int R, bool INV>
+E3_HD void fft_reg(cpx<T>* v);
+
+// In-register DFT of size R = P Q, P in {3, 5}, Q a power of two: P-point
+// DFTs over stride Q, twiddles exp(-/+ 2 pi i n2 k1 / R), Q-point FFTs over
+// contiguous runs, then the index map k1 + P k2 <- Q k1 + k2.
+template <typename T, int R, bool INV>
+E3_HD void fft_mixed(cpx<T>* v) {
+  constexpr int P = R % 3 == 0 ? 3 : 5, Q = R / P;
+  static_assert(is_pow2(Q), "mixed radix: R must be 3 or 5 times a power of two");
+#pragma unroll
+  for (int n2 = 0; n2 < Q; ++n2) {
+    cpx<T> x[P];
+#pragma unroll
+    for (int n1 = 0; n1 < P; ++n1) x[n1] = v[Q * n1 + n2];
+    dft_odd<T, P, INV>(x);
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) v[Q * k1 + n2] = rotR<T, R, INV>(x[k1], (n2 * k1) % R);
+  }
+  if constexpr (Q > 1) {
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) fft_reg<T, Q, INV>(v + Q * k1);
+  }
+  cpx<T> t[R];
+#pragma unroll
+  for (int k1 = 0; k1 < P; ++k1)
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) t[k1 + P * k2] = v[Q * k1 + k2];
+#pragma unroll
+  for (int i = 0; i < R; ++i) v[i] = t[i];
+}
+
+// In-register DFT of size R: iterative radix-2 DIT for powers of two <= 32,
+// fft_mixed for 3 or 5 times a power of two.
 template <typename T, int R, bool INV>
 E3_HD void fft_reg(cpx<T>* v) {
-  static_assert(R == 2 || R == 4 || R == 8 || R == 16 || R == 32, "radix must be a power of two <= 32");
+  if constexpr (!is_pow2(R)) {
+    fft_mixed<T, R, INV>(v);
+    return;
+  } else {
+  static_assert(R == 1 || R == 2 || R == 4 || R == 8 || R == 16 || R == 32, "radix must be a power of two <= 32");
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int j = bit_reverse<R>(i);
@@ -81,6 +192,7 @@ E3_HD void fft_reg(cpx<T>* v) {
         v[i + j + half] = a - b;
       }
     }
+  }
   }
 }
 
@@ -125,6 +237,21 @@ E3_FAST_CFG(double, 64, 8, 16, 16)
 E3_FAST_CFG(double, 128, 16, 16, 16)
 E3_FAST_CFG(double, 256, 16, 8, 8)
 E3_FAST_CFG(double, 512, 32, 8, 8)
+// 3 and 5 times a power of two (mixed-radix stage A)
+E3_FAST_CFG(float, 48, 12, 64, 64)
+E3_FAST_CFG(float, 80, 20, 64, 64)
+E3_FAST_CFG(float, 96, 24, 64, 64)
+E3_FAST_CFG(float, 160, 40, 32, 32)
+E3_FAST_CFG(float, 192, 24, 32, 32)
+E3_FAST_CFG(float, 320, 40, 16, 16)
+E3_FAST_CFG(float, 384, 48, 16, 16)
+E3_FAST_CFG(double, 48, 12, 32, 32)
+E3_FAST_CFG(double, 80, 20, 32, 32)
+E3_FAST_CFG(double, 96, 24, 32, 32)
+E3_FAST_CFG(double, 160, 40, 16, 16)
+E3_FAST_CFG(double, 192, 24, 16, 16)
+E3_FAST_CFG(double, 320, 40, 8, 8)
+E3_FAST_CFG(double, 384, 48, 8, 8)
 #undef E3_FAST_CFG
 
 // ---------------------------------------------------------------------------
@@ -220,6 +347,14 @@ template <> struct XRadix<256> {
   static constexpr int A = r[0], B = r[1], C = r[2];
 };
 template <> struct XRadix<512> { static constexpr int A = 8, B = 8, C = 8; };
+// 3 and 5 times a power of two: the odd factor sits in the last (register) stage
+template <> struct XRadix<48> { static constexpr int A = 4, B = 1, C = 12; };
+template <> struct XRadix<80> { static constexpr int A = 4, B = 1, C = 20; };
+template <> struct XRadix<96> { static constexpr int A = 4, B = 2, C = 12; };
+template <> struct XRadix<160> { static constexpr int A = 4, B = 2, C = 20; };
+template <> struct XRadix<192> { static constexpr int A = 4, B = 4, C = 12; };
+template <> struct XRadix<320> { static constexpr int A = 4, B = 4, C = 20; };
+template <> struct XRadix<384> { static constexpr int A = 4, B = 4, C = 24; };
 
 template <typename T, int L, int C, int TC_ = 128 / (int)sizeof(cpx<T>), bool STAGE_ = true, int WAVES_ = 2>
 struct FastX {
@@ -682,7 +817,9 @@ bool fast_s_len(Run& run, const SArgs& a) {
 
 template <typename T, int L, class Run>
 bool fast_x_len(Run& run, const XArgs& a) {
-  using X3 = FastX<T, L, 3, E3_X3_TC, E3_X3_STAGE, E3_X3_WAVES>;
+  // mixed-radix lengths hold more live values: one wave per SIMD keeps them
+  // in registers (VGPR + AGPR) instead of scratch
+  using X3 = FastX<T, L, 3, E3_X3_TC, E3_X3_STAGE, is_pow2(L) ? E3_X3_WAVES : 1>;
   using X1 = FastX<T, L, 1>;
   if (a.N[0] * 2 != L || a.L[2] % X3::TC != 0 || !a.perm) return false;
   if (a.C == 3) {
@@ -729,12 +866,32 @@ bool fast_z_len(Run& run, const ZArgs& a) {
     default: return false;                      \
   }
 
+// strided and z passes also cover 3 and 5 times a power of two
+#define E3_FAST_SWITCH_MIXED(FN, T, RUN, ARGS)  \
+  switch (ARGS.plan.L) {                        \
+    case 16: return FN<T, 16>(RUN, ARGS);       \
+    case 32: return FN<T, 32>(RUN, ARGS);       \
+    case 48: return FN<T, 48>(RUN, ARGS);       \
+    case 64: return FN<T, 64>(RUN, ARGS);       \
+    case 80: return FN<T, 80>(RUN, ARGS);       \
+    case 96: return FN<T, 96>(RUN, ARGS);       \
+    case 128: return FN<T, 128>(RUN, ARGS);     \
+    case 160: return FN<T, 160>(RUN, ARGS);     \
+    case 192: return FN<T, 192>(RUN, ARGS);     \
+    case 256: return FN<T, 256>(RUN, ARGS);     \
+    case 320: return FN<T, 320>(RUN, ARGS);     \
+    case 384: return FN<T, 384>(RUN, ARGS);     \
+    case 512: return FN<T, 512>(RUN, ARGS);     \
+    default: return false;                      \
+  }
+
 template <typename T, class Run>
-bool fast_s(Run& run, const SArgs& a) { E3_FAST_SWITCH(fast_s_len, T, run, a) }
+bool fast_s(Run& run, const SArgs& a) { E3_FAST_SWITCH_MIXED(fast_s_len, T, run, a) }
 template <typename T, class Run>
-bool fast_x(Run& run, const XArgs& a) { E3_FAST_SWITCH(fast_x_len, T, run, a) }
+bool fast_x(Run& run, const XArgs& a) { E3_FAST_SWITCH_MIXED(fast_x_len, T, run, a) }
 template <typename T, class Run>
-bool fast_z(Run& run, const ZArgs& a) { E3_FAST_SWITCH(fast_z_len, T, run, a) }
+bool fast_z(Run& run, const ZArgs& a) { E3_FAST_SWITCH_MIXED(fast_z_len, T, run, a) }
 #undef E3_FAST_SWITCH
+#undef E3_FAST_SWITCH_MIXED
 
 }  // namespace e3

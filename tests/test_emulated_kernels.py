@@ -114,14 +114,22 @@ FAST_CASES = [((8, 8, 8), (1.0, 1.0, 1.0), 4.0), ((16, 8, 4), (1.0, 0.8, 0.6), 2
               ((8, 16, 32), (1.0, 1.0, 1.0), 3.0), ((5, 8, 16), (1.0, 1.0, 1.0), 3.0),
               ((8, 4, 32), (1.0, 0.7, 1.3), 4.0), ((16, 6, 32), (1.0, 0.8, 0.6), 2.0),
               ((32, 8, 64), (1.0, 1.0, 1.0), 6.0), ((64, 4, 16), (1.0, 0.7, 1.3), 4.0),
-              ((128, 2, 8), (1.0, 0.8, 0.6), 2.0)]
+              ((128, 2, 8), (1.0, 0.8, 0.6), 2.0),
+              # 3 and 5 times a power of two: mixed-radix strided and z passes
+              ((24, 40, 48), (1.0, 0.9, 1.1), 3.0), ((8, 24, 40), (1.0, 1.0, 1.0), 5.0),
+              ((4, 96, 24), (0.6, 1.0, 0.8), 4.0), ((2, 8, 96), (0.5, 0.7, 1.0), 6.0),
+              ((4, 4, 160), (0.5, 0.5, 1.2), 3.0), ((2, 192, 4), (0.4, 1.0, 0.5), 2.0),
+              ((48, 4, 16), (1.0, 0.5, 0.6), 3.0), ((40, 6, 8), (1.0, 0.6, 0.4), 4.0),
+              ((80, 2, 8), (1.0, 0.4, 0.3), 5.0), ((96, 3, 16), (1.0, 0.3, 0.5), 3.0),
+              ((160, 2, 8), (1.0, 0.2, 0.2), 2.0), ((192, 2, 8), (1.0, 0.2, 0.2), 2.0)]
 
 
 @pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("single", 1e-6)])
 @pytest.mark.parametrize("N,L,k", FAST_CASES)
-def test_fast_power_of_two_kernels_match_oracle(lib, N, L, k, prec, tol):
-    """Power-of-two doubled axes take the register-resident two-stage kernels
-    (e3_fast.hpp); mixed shapes combine them with the generic passes."""
+def test_fast_kernels_match_oracle(lib, N, L, k, prec, tol):
+    """Doubled axes of 2^k, 3 2^k or 5 2^k points take the register-resident
+    two-stage kernels (e3_fast.hpp); other axes combine them with the generic
+    passes."""
     lib.emu_set_fast(1)
     before = [lib.emu_fast_calls(i) for i in range(4)]
     chi = _rand((3, 3) + N, 8) * 0.3
@@ -138,9 +146,12 @@ def test_fast_power_of_two_kernels_match_oracle(lib, N, L, k, prec, tol):
     assert _rel(opa.matvec(ua), vie.acoustic_matvec(ca, ua, vie.acoustic_kernel_hat(N, L, k))) < tol
     assert _rel(opa.rmatvec(ua), vie.acoustic_rmatvec(ca, ua, vie.acoustic_kernel_hat(N, L, k, adjoint=True))) < tol
     used = [lib.emu_fast_calls(i) - before[i] for i in range(4)]
-    if 2 * N[2] >= 16:
+    fast_len = {16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512}
+    if 2 * N[2] in fast_len:
         assert used[2] > 0 and used[3] > 0      # z passes take the fast kernels
-    if N[2] % 8 == 0 and N[0] >= 8:
+    if 2 * N[1] in fast_len:
+        assert used[0] > 0                       # strided y passes on the fast kernel
+    if N[2] % 8 == 0 and 2 * N[0] in fast_len:
         assert used[1] > 0                       # x pass on the fast kernel
 
 
