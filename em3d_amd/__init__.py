@@ -7,19 +7,19 @@ hand-written gfx950 HIP kernels of ``libem3d_hip.so``.
 """
 from __future__ import annotations
 
-from . import acoustics, geometry
+from . import acoustics, geometry, precond
 from .backend import Backend
 from .dtypes import Precision
 from .grid import Grid
 from .operator import Operator, PreparedEMKernel
 from .problem import Problem
 from .refraction import apply_refraction, cylinder_refraction, ellipsis_refraction, step_refraction
-from .solvers import SIM, BaseSolver, BiCGStab, MixedBiCGStab, SolverConfig, SolverResult, TwoStep
+from .solvers import SIM, BaseSolver, BiCGStab, FlexibleBiCGStab, MixedBiCGStab, SolverConfig, SolverResult, TwoStep
 from .wave import flat_wave_vec
 
 __version__ = "0.1.0"
 
 __all__ = ["Backend", "Precision", "Grid", "Problem", "Operator", "PreparedEMKernel", "apply_refraction",
            "cylinder_refraction", "ellipsis_refraction", "step_refraction", "flat_wave_vec", "geometry",
-           "acoustics", "BaseSolver", "BiCGStab", "MixedBiCGStab", "SIM", "SolverConfig", "SolverResult", "TwoStep",
+           "acoustics", "BaseSolver", "BiCGStab", "FlexibleBiCGStab", "MixedBiCGStab", "SIM", "SolverConfig", "SolverResult", "TwoStep",
            "__version__"]

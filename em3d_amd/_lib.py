@@ -21,6 +21,17 @@ class SolverResultC(C.Structure):
                 ("history_len", C.c_int), ("true_final_residual", C.c_double)]
 
 
+class FnoConfig(C.Structure):
+    _fields_ = [("grid", C.c_int), ("channels", C.c_int), ("modes", C.c_int), ("layers", C.c_int),
+                ("in_channels", C.c_int), ("out_channels", C.c_int)]
+
+
+class FnoModel(C.Structure):
+    _fields_ = [("config", FnoConfig), ("factor", C.c_int), ("params", C.c_void_p), ("params_bf16", C.c_void_p),
+                ("work", C.c_void_p), ("features", C.c_void_p), ("chi_features", C.c_void_p),
+                ("chi_channels", C.c_int)]
+
+
 STATUS = {0: "converged", 1: "nonfinite", 2: "max_operator_actions", 3: "breakdown_rho", 4: "breakdown_omega",
           5: "breakdown_alpha", 6: "breakdown_t", 7: "divergence_guard", 8: "max_iter"}
 
@@ -44,6 +55,24 @@ SIGNATURES = {
     "em3d_plane_waves": (_I, [_I, _I, _I, _VP, _VP, _VP, _I, _I, _VP, _D, _I, _VP, _VP]),
     "em3d_ellipsoid_contrast": (_I, [_I, _I, _I, _VP, _VP, _VP, _I, _I, _VP, _VP, _D, _I, _VP, _VP]),
     "em3d_eta_inclusions": (_I, [_I, _I, _I, _VP, _VP, _VP, _I, _I, _VP, _D, _D, _VP, _VP]),
+    "em3d_fno_param_count": (_LL, [C.POINTER(FnoConfig)]),
+    "em3d_fno_work_bytes": (_SZ, [C.POINTER(FnoConfig), _I]),
+    "em3d_fno_init_work": (_I, [C.POINTER(FnoConfig), _I, _VP, _VP]),
+    "em3d_fno_forward": (_I, [C.POINTER(FnoConfig), _VP, _VP, _VP, _VP, _VP, _I, _VP]),
+    "em3d_fno_backward": (_I, [C.POINTER(FnoConfig), _VP, _VP, _VP, _VP, _VP, _I, _VP]),
+    "em3d_fno_loss_work_bytes": (_SZ, [_I]),
+    "em3d_fno_loss": (_I, [_I, _LL, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "em3d_fno_adam": (_I, [_LL, _VP, _VP, _VP, _VP, _VP, _D, _D, _D, _D, _D, _I, _VP]),
+    "em3d_fno_to_bf16": (_I, [_LL, _VP, _VP, _VP]),
+    "em3d_fno_features_bytes": (_SZ, [C.POINTER(FnoConfig), _I]),
+    "em3d_fno_scale_work_bytes": (_SZ, [_I]),
+    "em3d_fno_scale": (_I, [_I, _LL, _I, _VP, _VP, _VP, _VP]),
+    "em3d_fno_restrict": (_I, [_I, _I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _I, _VP, _VP]),
+    "em3d_fno_chi_features": (_I, [_I, _I, _I, _I, _I, _VP, _I, _I, _VP, _VP]),
+    "em3d_fno_precondition": (_I, [C.POINTER(FnoModel), _I, _I, _I, _I, _I, _I, _VP, _VP, _VP]),
+    "em3d_bicgstab_pc_work_bytes": (_SZ, [_VP]),
+    "em3d_bicgstab_pc": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, C.POINTER(BiCGStabParams), _VP, _VP, _I,
+                              C.POINTER(SolverResultC), C.POINTER(FnoModel), _VP]),
     "em3d_reduce_work_bytes": (_SZ, []),
     "em3d_dotc": (_I, [_I, _LL, _VP, _VP, _VP, _VP, _VP]),
     "em3d_axpbypcz": (_I, [_I, _LL, _D, _D, _VP, _D, _D, _VP, _D, _D, _VP, _VP, _VP]),

@@ -27,9 +27,9 @@ class AcousticOperator(_FieldApplier):
         """``u - B(chi u)``."""
         return self._apply(u, self._K_hat, CHI_ISO, self._chi, False)
 
-    def device_bicgstab(self, rhs, cfg, batched: bool = False):
+    def device_bicgstab(self, rhs, cfg, batched: bool = False, precond=None):
         """BiCGStab with all iteration scalars on the device (libem3d_hip)."""
-        return self._device_bicgstab(rhs, cfg, self._K_hat, CHI_ISO, self._chi, batched)
+        return self._device_bicgstab(rhs, cfg, self._K_hat, CHI_ISO, self._chi, batched, precond)
 
     def rmatvec(self, v):
         """``v - conj(chi) B^H v``."""

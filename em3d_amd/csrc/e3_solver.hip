@@ -20,6 +20,7 @@
 
 #include "e3_common.hpp"
 #include "e3_error.hpp"
+#include "e3_fno.hpp"
 #include "e3_plan.hpp"
 
 using namespace e3;
@@ -186,24 +187,26 @@ __global__ __launch_bounds__(kRT) void k_s(long long n, cpx<T>* s, const cpx<T>*
   if (threadIdx.x == 0) part[((size_t)b * kRB + blockIdx.x) * 4] = acc;
 }
 
-// u_alpha = u + alpha p; then (no halt) u = u_alpha + omega s, r = s - omega t, part = sum |r|^2
+// u_alpha = u + alpha pu; then (no halt) u = u_alpha + omega su, r = s - omega t, part = sum |r|^2
+// (pu, su: the search directions after the right preconditioner, p and s without one)
 template <typename T>
-__global__ __launch_bounds__(kRT) void k_ur(long long n, cpx<T>* u, const cpx<T>* p, const cpx<T>* s, cpx<T>* r,
-                                            const cpx<T>* t, const BiState* st, double* part) {
+__global__ __launch_bounds__(kRT) void k_ur(long long n, cpx<T>* u, const cpx<T>* pu, const cpx<T>* su,
+                                            const cpx<T>* s, cpx<T>* r, const cpx<T>* t, const BiState* st,
+                                            double* part) {
   const int b = blockIdx.y;
   if (st[b].done) return;
   const long long off = (long long)b * n;
-  u += off; p += off; s += off; r += off; t += off;
+  u += off; pu += off; su += off; s += off; r += off; t += off;
   const bool h = st[b].halt != 0;
   const cpx<T> al = cast<T>(st[b].alpha), om = cast<T>(st[b].omega);
   double acc = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const cpx<T> ua = u[i] + al * p[i];
+    const cpx<T> ua = u[i] + al * pu[i];
     if (h) {
       u[i] = ua;
     } else {
       const cpx<T> si = s[i];
-      u[i] = ua + om * si;
+      u[i] = ua + om * su[i];
       const cpx<T> x = si - om * t[i];
       r[i] = x;
       acc += (double)x.re * x.re + (double)x.im * x.im;
@@ -376,6 +379,7 @@ long long field_elems(const em3d_plan_s* plan) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // solver scratch: r, r_hat, p, v, s, t (batch fields each) | partials | states | group | rel0
+// | preconditioned p and s (preconditioned solves only)
 struct Work {
   void* r;
   void* rh;
@@ -387,15 +391,21 @@ struct Work {
   BiState* st;
   Group* g;
   double* rel0;
+  void* ph;
+  void* sh;
 };
 
 size_t part_bytes(int B) { return align256((size_t)B * kRB * 4 * sizeof(double)); }
+
+size_t work_total(size_t vb, int B);
 
 Work carve(void* base, size_t vb, int B) {
   Work w;
   char* c = (char*)base;
   const size_t a = align256(vb);
   w.r = c; w.rh = c + a; w.p = c + 2 * a; w.v = c + 3 * a; w.s = c + 4 * a; w.t = c + 5 * a;
+  w.ph = (char*)base + work_total(vb, B);
+  w.sh = (char*)w.ph + a;
   c += 6 * a;
   w.part = (double*)c;
   c += part_bytes(B);
@@ -414,15 +424,23 @@ size_t work_total(size_t vb, int B) {
 
 template <typename T>
 void enqueue_iteration(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, void* u, void* work,
-                       const Work& w, long long n, int nb, int B, Hist H, hipStream_t stream, int* fail) {
+                       const Work& w, long long n, int nb, int B, Hist H, const em3d_fno_model* pc,
+                       hipStream_t stream, int* fail) {
   const dim3 gv(nb, B), gd(B);
   const int* all_done = &w.g->all_done;
   const int* all_skip_t = &w.g->all_skip_t;
+  const Plan& P = plan->p;
+  void* pu = pc ? w.ph : w.p;   // search directions the matvecs and the update use
+  void* su = pc ? w.sh : w.s;
   hipLaunchKernelGGL(k_dot<T>, gv, dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.r, w.part, w.st, 0);
   hipLaunchKernelGGL(k_rho, gd, dim3(kRT), 0, stream, w.part, nb, w.st);
   hipLaunchKernelGGL(k_p<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)w.p, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st);
   hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
-  if (apply_plan(plan, khat, chi, chi_kind, w.p, w.v, work, stream, false, "em3d_bicgstab matvec", all_done)) {
+  if (pc && fno_precond_apply(pc, B, P.C, P.N[0], P.N[1], P.N[2], P.prec, w.p, w.ph, all_done, stream)) {
+    *fail = 1;
+    return;
+  }
+  if (apply_plan(plan, khat, chi, chi_kind, pu, w.v, work, stream, false, "em3d_bicgstab matvec", all_done)) {
     *fail = 1;
     return;
   }
@@ -432,14 +450,18 @@ void enqueue_iteration(em3d_plan_t plan, const void* khat, const void* chi, int 
                      w.part);
   hipLaunchKernelGGL(k_sfin, gd, dim3(kRT), 0, stream, w.part, nb, w.st, H);
   hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
-  if (apply_plan(plan, khat, chi, chi_kind, w.s, w.t, work, stream, false, "em3d_bicgstab matvec", all_skip_t)) {
+  if (pc && fno_precond_apply(pc, B, P.C, P.N[0], P.N[1], P.N[2], P.prec, w.s, w.sh, all_skip_t, stream)) {
+    *fail = 1;
+    return;
+  }
+  if (apply_plan(plan, khat, chi, chi_kind, su, w.t, work, stream, false, "em3d_bicgstab matvec", all_skip_t)) {
     *fail = 1;
     return;
   }
   hipLaunchKernelGGL(k_dot2<T>, gv, dim3(kRT), 0, stream, n, (const cpx<T>*)w.t, (const cpx<T>*)w.s, w.part, w.st);
   hipLaunchKernelGGL(k_omega, gd, dim3(kRT), 0, stream, w.part, nb, w.st, H);
-  hipLaunchKernelGGL(k_ur<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)u, (const cpx<T>*)w.p, (const cpx<T>*)w.s,
-                     (cpx<T>*)w.r, (const cpx<T>*)w.t, w.st, w.part);
+  hipLaunchKernelGGL(k_ur<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)u, (const cpx<T>*)pu, (const cpx<T>*)su,
+                     (const cpx<T>*)w.s, (cpx<T>*)w.r, (const cpx<T>*)w.t, w.st, w.part);
   hipLaunchKernelGGL(k_rfin, gd, dim3(kRT), 0, stream, w.part, nb, w.st, H);
 }
 
@@ -453,9 +475,20 @@ size_t em3d_bicgstab_work_bytes(em3d_plan_t plan) {
   return work_total((size_t)B * field_elems(plan) * elem_bytes(plan->p.prec), B);
 }
 
-int em3d_bicgstab(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* rhs, void* u,
+size_t em3d_bicgstab_pc_work_bytes(em3d_plan_t plan) {
+  if (!plan) return 0;
+  const size_t vb = (size_t)plan->p.batch * field_elems(plan) * elem_bytes(plan->p.prec);
+  return em3d_bicgstab_work_bytes(plan) + 2 * align256(vb);
+}
+
+}  // extern "C"
+
+namespace {
+
+int bicgstab_impl(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* rhs, void* u,
                   void* work, void* solver_work, const em3d_bicgstab_params* prm, double* history,
-                  long long* action_counts, int history_cap, em3d_solver_result* res, hipStream_t stream) {
+                  long long* action_counts, int history_cap, em3d_solver_result* res, const em3d_fno_model* pc,
+                  hipStream_t stream) {
   if (!plan || !khat || !rhs || !u || !work || !solver_work || !prm || !res || !history || !action_counts)
     return sfail("em3d_bicgstab: null argument");
   if (history_cap < 1) return sfail("em3d_bicgstab: history_cap must be >= 1");
@@ -545,9 +578,9 @@ int em3d_bicgstab(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
       for (int k = 0; k < chunk && it0 + k < prm->max_iter; ++k) {
         int fail = 0;
         if (prec == PREC_DOUBLE)
-          enqueue_iteration<double>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, stream, &fail);
+          enqueue_iteration<double>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, pc, stream, &fail);
         else
-          enqueue_iteration<float>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, stream, &fail);
+          enqueue_iteration<float>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, pc, stream, &fail);
         if (fail) return 1;
         if ((e = hipGetLastError()) != hipSuccess) return chk(e, "iteration launch");
       }
@@ -571,6 +604,34 @@ int em3d_bicgstab(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
     res[b].true_final_residual = s.last_rel;
   }
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int em3d_bicgstab(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* rhs, void* u,
+                  void* work, void* solver_work, const em3d_bicgstab_params* prm, double* history,
+                  long long* action_counts, int history_cap, em3d_solver_result* res, hipStream_t stream) {
+  return bicgstab_impl(plan, khat, chi, chi_kind, rhs, u, work, solver_work, prm, history, action_counts, history_cap,
+                       res, nullptr, stream);
+}
+
+int em3d_bicgstab_pc(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* rhs, void* u,
+                     void* work, void* solver_work, const em3d_bicgstab_params* prm, double* history,
+                     long long* action_counts, int history_cap, em3d_solver_result* res,
+                     const em3d_fno_model* precond, hipStream_t stream) {
+  if (!precond) return sfail("em3d_bicgstab_pc: null preconditioner");
+  const std::string why = fno_check(&precond->config);
+  if (!why.empty()) return sfail("em3d_bicgstab_pc: " + why);
+  const Plan& P = plan ? plan->p : Plan{};
+  if (!plan || precond->factor < 1 || P.N[0] != precond->config.grid * precond->factor ||
+      P.N[1] != precond->config.grid * precond->factor || P.N[2] != precond->config.grid * precond->factor)
+    return sfail("em3d_bicgstab_pc: the plan grid must be (config.grid * factor)^3");
+  if (precond->config.in_channels != 2 * P.C + precond->chi_channels || precond->config.out_channels != 2 * P.C)
+    return sfail("em3d_bicgstab_pc: in_channels must be 2 ncomp + chi_channels, out_channels 2 ncomp");
+  return bicgstab_impl(plan, khat, chi, chi_kind, rhs, u, work, solver_work, prm, history, action_counts, history_cap,
+                       res, precond, stream);
 }
 
 }  // extern "C"

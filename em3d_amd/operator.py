@@ -307,10 +307,11 @@ class _FieldApplier:
             raise ValueError(f"field shape {tuple(t.shape)} != expected {fs} (or (batch,) + {fs})")
         return t.contiguous(), batch
 
-    def _device_bicgstab(self, rhs, cfg, khat, chi_kind, chi, batched: bool = False):
+    def _device_bicgstab(self, rhs, cfg, khat, chi_kind, chi, batched: bool = False, precond=None):
         """Device-resident BiCGStab of libem3d_hip on this operator.  ``batched``:
         rhs carries a leading batch axis and a list of results (one per RHS) is
-        returned; every RHS runs its own recurrence and status."""
+        returned; every RHS runs its own recurrence and status.  ``precond``: an
+        FNOPreconditioner attached to this operator (right preconditioning)."""
         import torch
 
         from .solvers.base import SolverResult
@@ -320,7 +321,10 @@ class _FieldApplier:
         be = self._be
         plan = HipPlan.get(self._N, self._ncomp, batch, be.precision, be.device_index)
         lib = _lib.lib()
-        sw = torch.empty(int(lib.em3d_bicgstab_work_bytes(plan.handle)), dtype=torch.uint8, device=be.torch_device)
+        if precond is not None and precond.operator is not self:
+            raise ValueError("the preconditioner is attached to another operator")
+        wb = lib.em3d_bicgstab_pc_work_bytes if precond is not None else lib.em3d_bicgstab_work_bytes
+        sw = torch.empty(int(wb(plan.handle)), dtype=torch.uint8, device=be.torch_device)
         cap = int(cfg.max_iter) + 2
         hist = torch.zeros((batch, cap), dtype=torch.float64, device=be.torch_device)
         acts = torch.zeros((batch, cap), dtype=torch.int64, device=be.torch_device)
@@ -330,10 +334,14 @@ class _FieldApplier:
                                   -1 if cfg.max_operator_actions is None else int(cfg.max_operator_actions),
                                   int(getattr(cfg, "chunk", 4) or 4))
         res = (_lib.SolverResultC * batch)()
-        _lib.check(lib.em3d_bicgstab(plan.handle, khat.data_ptr(), chi.data_ptr() if chi is not None else None,
-                                     chi_kind, t.data_ptr(), u.data_ptr(), self._workspace(plan).data_ptr(),
-                                     sw.data_ptr(), C.byref(prm), hist.data_ptr(), acts.data_ptr(), cap,
-                                     res, _lib.stream_handle(be.torch_device)), "em3d_bicgstab")
+        args = (plan.handle, khat.data_ptr(), chi.data_ptr() if chi is not None else None, chi_kind, t.data_ptr(),
+                u.data_ptr(), self._workspace(plan).data_ptr(), sw.data_ptr(), C.byref(prm), hist.data_ptr(),
+                acts.data_ptr(), cap, res)
+        if precond is None:
+            _lib.check(lib.em3d_bicgstab(*args, _lib.stream_handle(be.torch_device)), "em3d_bicgstab")
+        else:
+            _lib.check(lib.em3d_bicgstab_pc(*args, C.byref(precond.model(batch)),
+                                            _lib.stream_handle(be.torch_device)), "em3d_bicgstab_pc")
         hist_h, acts_h = hist.cpu(), acts.cpu()
         out = []
         for b in range(batch):
@@ -381,8 +389,8 @@ class _SingleCompanion(_FieldApplier):
     def rmatvec(self, field):
         return self._apply(field, self._khat, self._chi_kind, self._chi, True)
 
-    def device_bicgstab(self, rhs, cfg, batched: bool = False):
-        return self._device_bicgstab(rhs, cfg, self._khat, self._chi_kind, self._chi, batched)
+    def device_bicgstab(self, rhs, cfg, batched: bool = False, precond=None):
+        return self._device_bicgstab(rhs, cfg, self._khat, self._chi_kind, self._chi, batched, precond)
 
 
 class Operator(_FieldApplier):
@@ -423,10 +431,10 @@ class Operator(_FieldApplier):
     def matvec(self, field):
         return self._apply(field, self._prepared_kernel.khat_octant, self._chi_kind, self._chi, False)
 
-    def device_bicgstab(self, rhs, cfg, batched: bool = False):
+    def device_bicgstab(self, rhs, cfg, batched: bool = False, precond=None):
         """BiCGStab with all iteration scalars on the device (libem3d_hip)."""
         return self._device_bicgstab(rhs, cfg, self._prepared_kernel.khat_octant, self._chi_kind, self._chi,
-                                     batched)
+                                     batched, precond)
 
     def rmatvec(self, field):
         if not self._prepared_kernel.supports_adjoint:

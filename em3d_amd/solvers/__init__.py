@@ -1,7 +1,8 @@
 from .base import BaseSolver, SolverConfig, SolverResult
 from .bicgstab import BiCGStab
+from .flexible import FlexibleBiCGStab
 from .refine import MixedBiCGStab
 from .sim import SIM
 from .twostep import TwoStep
 
-__all__ = ["BaseSolver", "SolverConfig", "SolverResult", "SIM", "BiCGStab", "MixedBiCGStab", "TwoStep"]
+__all__ = ["BaseSolver", "SolverConfig", "SolverResult", "SIM", "BiCGStab", "FlexibleBiCGStab", "MixedBiCGStab", "TwoStep"]

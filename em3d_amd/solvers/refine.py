@@ -14,6 +14,9 @@ experiments/stationary_validation.py:172-180) and the update ``x += d``:
              r = f - A64 x                         (FP64 matvec)
     until |r| / |f| < max(rtol, atol / |f|)
 
+With a ``preconditioner`` (a trained FNO) the inner solves are the
+right-preconditioned device BiCGStab of ``FlexibleBiCGStab``.
+
 Each outer step divides the residual by about ``inner_rtol``, so rtol 1e-8
 takes two or three steps.  When a step fails to reduce the true residual by
 ``stagnation`` (an inner breakdown, or an operator whose complex64
@@ -46,7 +49,7 @@ class MixedBiCGStab:
     """complex64 BiCGStab corrections refined against the FP64 operator."""
 
     def __init__(self, config: SolverConfig, *, inner_rtol: float = 1e-5, max_outer: int = 12,
-                 stagnation: float = 0.5, inner_max_iter: int | None = None):
+                 stagnation: float = 0.5, inner_max_iter: int | None = None, preconditioner=None):
         if not (0.0 < inner_rtol < 1.0):
             raise ValueError(f"inner_rtol must lie in (0, 1), got {inner_rtol}")
         if max_outer < 1:
@@ -56,6 +59,7 @@ class MixedBiCGStab:
         self.max_outer = int(max_outer)
         self.stagnation = float(stagnation)
         self.inner_max_iter = inner_max_iter
+        self.preconditioner = preconditioner   # FNO: inner solves become right-preconditioned
 
     def solve(self, operator, rhs) -> SolverResult:
         cfg = self.cfg
@@ -66,6 +70,10 @@ class MixedBiCGStab:
         if batch != 1:
             raise ValueError("MixedBiCGStab solves one right-hand side")
         comp = operator.single_precision_companion()
+        pc = None
+        if self.preconditioner is not None:
+            from .flexible import attach
+            pc = attach(self.preconditioner, comp)
         fn = norm(f)
         x = torch.zeros_like(f)
         if fn == 0.0:
@@ -88,7 +96,7 @@ class MixedBiCGStab:
             # solve no further than the remaining reduction asks (with a margin)
             want = max(self.inner_rtol, 0.5 * tol / rel)
             icfg = SolverConfig(max_iter=budget, rtol=min(want, 0.5), chunk=cfg.chunk)
-            ires = comp.device_bicgstab(r32, icfg)
+            ires = comp.device_bicgstab(r32, icfg, precond=pc)
             inner_it += ires.iterations
             mv32 += ires.matvec_count
             inner_status.append(ires.status)

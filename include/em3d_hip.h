@@ -173,6 +173,85 @@ int em3d_eta_inclusions(int nx, int ny, int nz, const void* ax, const void* ay, 
                         int n_inclusions, const double* inclusions, double outside_re, double outside_im, void* eta,
                         hipStream_t stream);
 
+/* ---- learned preconditioner: Fourier neural operator (FNO) ----------------
+ * No counterpart in the reference (its solvers are unpreconditioned,
+ * solvers/bicgstab.py:12-211).  A network on a grid^3 coarse grid with
+ * `channels` hidden channels, `layers` Fourier layers keeping `modes` modes
+ * per axis (k in [-modes/2, modes/2)), pointwise channel mixing on bf16 MFMA
+ * with fp32 accumulation.  Parameters: fp32 master array of
+ * em3d_fno_param_count() floats with a bf16 copy (unsigned short) the
+ * forward pass reads; tensors are channels-last [samples][cells][channels]
+ * fp32.  work: em3d_fno_work_bytes(), initialised once by
+ * em3d_fno_init_work() (twiddle table); a forward call keeps the activations
+ * em3d_fno_backward() of the same samples needs. */
+typedef struct {
+  int grid, channels, modes, layers, in_channels, out_channels;
+} em3d_fno_config;
+
+long long em3d_fno_param_count(const em3d_fno_config* config);
+size_t em3d_fno_work_bytes(const em3d_fno_config* config, int samples);
+int em3d_fno_init_work(const em3d_fno_config* config, int samples, void* work, hipStream_t stream);
+int em3d_fno_forward(const em3d_fno_config* config, const float* params, const unsigned short* params_bf16,
+                     const float* x, float* y, void* work, int samples, hipStream_t stream);
+/* grad (fp32, param layout) += dL/dparams for dL/dy = dy */
+int em3d_fno_backward(const em3d_fno_config* config, const unsigned short* params_bf16, const float* x,
+                      const float* dy, float* grad, void* work, int samples, hipStream_t stream);
+/* loss (device double, accumulated) += sum_s |y_s - t_s|^2 / |t_s|^2;
+ * dy = gradient of (that sum / samples); work: em3d_fno_loss_work_bytes() */
+size_t em3d_fno_loss_work_bytes(int samples);
+int em3d_fno_loss(int samples, long long per_sample, const float* y, const float* target, float* dy, double* loss,
+                  double* work, hipStream_t stream);
+int em3d_fno_adam(long long n, float* params, unsigned short* params_bf16, const float* grad, float* m, float* v,
+                  double lr, double beta1, double beta2, double eps, double weight_decay, int step,
+                  hipStream_t stream);
+int em3d_fno_to_bf16(long long n, const float* params, unsigned short* params_bf16, hipStream_t stream);
+
+/* Preconditioner M r = r + s P(y),  y = FNO(R r / s, chi features),
+ * s = |r| / sqrt(ncomp N^3) per right-hand side, R = mean over factor^3
+ * fine cells, P = copy to them.  Grids are (grid * factor)^3.
+ * chi_features: device [cells][chi_channels] from em3d_fno_chi_features
+ * (means of the diagonal of chi: 6 channels EM, 2 acoustic);
+ * features: em3d_fno_features_bytes(config, batch); work as above for
+ * `batch` samples. */
+typedef struct {
+  em3d_fno_config config;
+  int factor;
+  const float* params;
+  const unsigned short* params_bf16;
+  void* work;
+  float* features;
+  const float* chi_features;
+  int chi_channels;
+} em3d_fno_model;
+
+size_t em3d_fno_features_bytes(const em3d_fno_config* config, int samples);
+/* scale[b] = |field_b| / sqrt(n) (1 when zero) for batch fields of n complex
+ * values; work: em3d_fno_scale_work_bytes(batch). */
+size_t em3d_fno_scale_work_bytes(int batch);
+int em3d_fno_scale(int batch, long long n, int precision, const void* field, double* scale, void* work,
+                   hipStream_t stream);
+/* out[b][cell] = (mean over the cell's factor^3 fine cells of field_b / scale[b]
+ * as re, im per component, then the chi_channels chi features) */
+int em3d_fno_restrict(int batch, int ncomp, int nx, int ny, int nz, int factor, int precision, const void* field,
+                      const double* scale, const float* chi_features, int chi_channels, float* out,
+                      hipStream_t stream);
+int em3d_fno_chi_features(int nx, int ny, int nz, int factor, int precision, const void* chi, int chi_kind,
+                          int chi_channels, float* out, hipStream_t stream);
+int em3d_fno_precondition(const em3d_fno_model* model, int batch, int ncomp, int nx, int ny, int nz, int precision,
+                          const void* r, void* out, hipStream_t stream);
+
+/* em3d_bicgstab with the FNO as right preconditioner (flexible form: the
+ * preconditioned directions M p and M s are kept and enter the update, so
+ * the preconditioner may be any fixed nonlinear map).  Same arguments,
+ * statuses and histories as em3d_bicgstab; solver_work holds
+ * em3d_bicgstab_pc_work_bytes(); the model's work and features are sized for
+ * the plan's batch.  No counterpart in the reference. */
+size_t em3d_bicgstab_pc_work_bytes(em3d_plan_t plan);
+int em3d_bicgstab_pc(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* rhs, void* u,
+                     void* work, void* solver_work, const em3d_bicgstab_params* params, double* history,
+                     long long* action_counts, int history_cap, em3d_solver_result* result,
+                     const em3d_fno_model* precond, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
