@@ -65,7 +65,29 @@ def _timed(fn):
     return out, time.perf_counter() - t0
 
 
-def solve_section(em, N, k0s):
+def train_preconditioner(em, n_materials, steps, n_train):
+    """The learned preconditioner: FNO on a 32^3 coarse grid (32 channels, 16
+    modes per axis, 4 layers), trained on the preconditioned-residual loss
+    over residuals of seeded random materials solved at n_train^3."""
+    import torch
+    from em3d_amd import learn
+    from em3d_amd.precond import FNO, FNOConfig
+    be = em.Backend.hip(em.Precision.SINGLE)
+    fno = FNO(FNOConfig(grid=32, channels=32, modes=16, layers=4), 3, 3, material_channels=7,
+              device=torch.device("cuda", 0), seed=0)
+    items, t_set = _timed(lambda: learn.make_residual_set(fno, range(n_materials), n_train, be,
+                                                          iterations=(0, 2, 6), noise=True))
+    hist, t_train = _timed(lambda: learn.train_residual(fno, items, steps=steps, batch=8, lr=1e-3))
+    del items
+    torch.cuda.empty_cache()
+    info = {"materials": n_materials, "train_grid": n_train, "steps": steps, "parameters": fno.n_params,
+            "pairs_seconds": round(t_set, 2), "train_seconds": round(t_train, 2),
+            "loss_first": round(hist[0], 4), "loss_last": round(hist[-1], 4)}
+    log(f"[bench] FNO trained: {info}")
+    return fno, info
+
+
+def solve_section(em, N, k0s, learned=True):
     """Time to true rtol 1e-8 at N^3 for each k0: plain complex128 BiCGStab
     vs complex64 BiCGStab with FP64 refinement.  Wall clock from solve()
     entry to exit on a synchronised device; the complex64 spectrum build of
@@ -74,6 +96,9 @@ def solve_section(em, N, k0s):
     be64 = em.Backend.hip(em.Precision.DOUBLE)
     cfg = em.SolverConfig(max_iter=1000, rtol=1e-8)
     out = {}
+    fno = None
+    if learned:
+        fno, out["learned_preconditioner_training"] = train_preconditioner(em, 200, 600, 64)
     for k0 in k0s:
         prob, _ = make_problem(em, be64, N, k0)
         op = em.Operator(prob)
@@ -96,6 +121,15 @@ def solve_section(em, N, k0s):
             "solution_rel_diff": float(torch.linalg.vector_norm(mixed.u - plain.u) / torch.linalg.vector_norm(plain.u)),
             "speedup_mixed_vs_fp64": round(t_plain / t_mixed, 3),
         }
+        if fno is not None:
+            lm, t_lm = _timed(lambda: em.MixedBiCGStab(cfg, preconditioner=fno).solve(op, f))
+            rec["learned_pc_mixed_bicgstab"] = {
+                "seconds": round(t_lm, 4), "iterations": lm.iterations, "fp32_matvecs": lm.extras["fp32_matvecs"],
+                "fp64_matvecs": lm.extras["fp64_matvecs"], "status": lm.status,
+                "inner_status": lm.extras["inner_status"], "true_residual": _true_residual(op, lm.u, f)}
+            rec["speedup_learned_pc_mixed_vs_fp64"] = round(t_plain / t_lm, 3)
+            log(f"[bench] k0={k0}: learned-PC mixed {t_lm:.3f} s ({lm.iterations} it, {lm.status}, "
+                f"inner {lm.extras['inner_status']})")
         log(f"[bench] k0={k0}: fp64 {t_plain:.3f} s ({plain.iterations} it, {plain.status}), "
             f"mixed {t_mixed:.3f} s ({mixed.iterations} it, {mixed.status}), x{t_plain / t_mixed:.2f}")
         out[f"k0={k0:g}"] = rec
@@ -196,6 +230,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned preconditioner")
     ap.add_argument("--batch-n", type=int, default=96)
     ap.add_argument("--batch-dirs", type=int, default=64)
     ap.add_argument("--k0s", type=float, nargs="*", default=[4.0, 8.0, 12.0],
@@ -268,7 +303,7 @@ def main():
 
     solve = None
     if not args.no_solve:
-        solve = solve_section(em, N, args.k0s)
+        solve = solve_section(em, N, args.k0s, learned=not args.no_learned)
 
     batch = None
     if not args.no_batch:

@@ -23,7 +23,7 @@ class SolverResultC(C.Structure):
 
 class FnoConfig(C.Structure):
     _fields_ = [("grid", C.c_int), ("channels", C.c_int), ("modes", C.c_int), ("layers", C.c_int),
-                ("in_channels", C.c_int), ("out_channels", C.c_int)]
+                ("in_channels", C.c_int), ("out_channels", C.c_int), ("material_channels", C.c_int)]
 
 
 class FnoModel(C.Structure):
@@ -68,7 +68,9 @@ SIGNATURES = {
     "em3d_fno_scale_work_bytes": (_SZ, [_I]),
     "em3d_fno_scale": (_I, [_I, _LL, _I, _VP, _VP, _VP, _VP]),
     "em3d_fno_restrict": (_I, [_I, _I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _I, _VP, _VP]),
-    "em3d_fno_chi_features": (_I, [_I, _I, _I, _I, _I, _VP, _I, _I, _VP, _VP]),
+    "em3d_fno_chi_features": (_I, [_I, _I, _I, _I, _I, _VP, _I, _I, _D, _VP, _VP]),
+    "em3d_fno_prolong": (_I, [_I, _I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
+    "em3d_fno_prolong_adjoint": (_I, [_I, _I, _I, _I, _I, _I, _I, _VP, _VP, _VP, _VP]),
     "em3d_fno_precondition": (_I, [C.POINTER(FnoModel), _I, _I, _I, _I, _I, _I, _VP, _VP, _VP]),
     "em3d_bicgstab_pc_work_bytes": (_SZ, [_VP]),
     "em3d_bicgstab_pc": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, C.POINTER(BiCGStabParams), _VP, _VP, _I,

@@ -5,20 +5,27 @@
 // No counterpart in the reference (its solvers are unpreconditioned,
 // src/em3d/solvers/bicgstab.py:12-211).
 //
-// Network on a G^3 coarse grid, channels last (cell-major, channel-minor):
-//   v0      = W_in x + b_in                                 (in -> H)
-//   v_{l+1} = gelu(W_l v_l + b_l + S_l v_l),  l < layers    (H -> H)
-//   y       = W_out v_L + b_out                             (H -> out)
+// Network on a G^3 coarse grid, channels last (cell-major, channel-minor).
+// It is linear in the residual and conditioned on the material through
+// multiplicative gates, so that the preconditioner is a fixed linear map for
+// a given operator (BiCGStab's recurrences assume one):
+//   v0      = W_in x                                   (in -> H; x: residual channels)
+//   g_l     = sigmoid(U_l m + d_l)                     (material channels -> H)
+//   z_l     = W_l v_l + S_l v_l
+//   v_{l+1} = g_l * z_l                                (l < layers)
+//   y       = W_out v_L                                (H -> out)
 //   S_l v   = Re( E+ (R_l . (E- v)) ) / G^3
 // E- is the forward DFT restricted to the modes k in [-M/2, M/2) per axis
 // (M^3 modes), E+ the matching inverse sum, R_l(k) a complex H x H matrix
-// per mode.  Pointwise channel mixing (W_in, W_l, W_out and the transposed
-// products of the backward pass) runs on bf16 MFMA with fp32 accumulation;
-// parameters are fp32 masters with a bf16 copy the forward pass reads.
+// per mode.  Pointwise channel mixing (W_in, W_l, U_l, W_out and the
+// transposed products of the backward pass) runs on bf16 MFMA with fp32
+// accumulation; parameters are fp32 masters with a bf16 copy the forward
+// pass reads.
 //
-// Preconditioner:  M r = r + s P( y(R r / s, chi) ),  s = ||r|| / sqrt(n):
-// R averages f^3 fine cells into one coarse cell, P copies a coarse value to
-// its f^3 fine cells.
+// Preconditioner:  M r = r + P( y(R r, m) ): R averages f^3 fine cells into
+// one coarse cell (scaled by s = |r| / sqrt(n), undone after), P is the
+// trilinear interpolation back to the fine grid; m holds the coarse means of
+// the diagonal of chi and the wave-number feature.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -45,32 +52,22 @@ __device__ __forceinline__ unsigned short to_bf16(float x) {
 }
 __device__ __forceinline__ float from_bf16(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
-constexpr float kGeluC = 0.7978845608028654f;   // sqrt(2 / pi)
-constexpr float kGeluA = 0.044715f;
-__device__ __forceinline__ float gelu(float x) {
-  const float t = tanhf(kGeluC * (x + kGeluA * x * x * x));
-  return 0.5f * x * (1.0f + t);
-}
-__device__ __forceinline__ float gelu_grad(float x) {
-  const float t = tanhf(kGeluC * (x + kGeluA * x * x * x));
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
-}
-
 unsigned grid1(long long n) {
   const long long b = (n + kT - 1) / kT;
   return (unsigned)(b < 1 ? 1 : (b > 1048576 ? 1048576 : b));
 }
 
 // ---- pointwise channel mixing on MFMA ---------------------------------------
-// Y[r][o] = act(sum_i A[r][i] W[o][i] + bias[o] + add[r][o]) for r < rows,
+// z[r][o] = sum_{i<K} A[r * lda + i] W[o][i] + bias[o] + add[r][o] for r < rows,
 // W bf16 row-major [cout][K] (transpose: W given as [K][cout], i.e. the
-// product with W^T).  Z (optional) receives the pre-activation.  One wave
-// computes a 16-row tile for all output columns, K in steps of 16:
-// v_mfma_f32_16x16x16_bf16 with lane l holding A[l%16][4(l/16)+i],
-// B[4(l/16)+i][l%16] and D[4(l/16)+i][l%16], i = 0..3.
-__global__ __launch_bounds__(kT) void k_mix(long long rows, int K, int cout, const float* A, const unsigned short* W,
-                                            int transpose, const float* bias, const float* add, float* Z, float* Y,
-                                            int act, const int* skip) {
+// product with W^T).  Y = z (act 0), gate[r][o] * z (act 1) or sigmoid(z)
+// (act 2); Z (optional) receives z.  One wave computes a 16-row tile for all
+// output columns, K in steps of 16: v_mfma_f32_16x16x16_bf16 with lane l
+// holding A[l%16][4(l/16)+i], B[4(l/16)+i][l%16] and D[4(l/16)+i][l%16].
+__global__ __launch_bounds__(kT) void k_mix(long long rows, int K, int lda, int cout, const float* A,
+                                            const unsigned short* W, int transpose, const float* bias,
+                                            const float* add, const float* gate, float* Z, float* Y, int act,
+                                            const int* skip) {
   if (skip && *skip) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const long long r0 = ((long long)blockIdx.x * (kT / 64) + wave) * 16;
@@ -84,7 +81,7 @@ __global__ __launch_bounds__(kT) void k_mix(long long rows, int K, int cout, con
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int k = k0 + lk + i;
-        const float av = (ra < rows && k < K) ? A[ra * K + k] : 0.f;
+        const float av = (ra < rows && k < K) ? A[ra * lda + k] : 0.f;
         a[i] = (short)to_bf16(av);
         const int c = c0 + lr;
         unsigned short bv = 0;
@@ -99,9 +96,10 @@ __global__ __launch_bounds__(kT) void k_mix(long long rows, int K, int cout, con
       for (int i = 0; i < 4; ++i) {
         const long long r = r0 + lk + i;
         if (r >= rows) continue;
-        float z = acc[i] + (bias ? bias[c] : 0.f) + (add ? add[r * cout + c] : 0.f);
-        if (Z) Z[r * cout + c] = z;
-        Y[r * cout + c] = act ? gelu(z) : z;
+        const long long o = r * cout + c;
+        const float z = acc[i] + (bias ? bias[c] : 0.f) + (add ? add[o] : 0.f);
+        if (Z) Z[o] = z;
+        Y[o] = act == 1 ? gate[o] * z : (act == 2 ? 1.f / (1.f + expf(-z)) : z);
       }
     }
   }
@@ -193,20 +191,22 @@ __global__ __launch_bounds__(kT) void k_specgrad(int S, int modes, int H, const 
   }
 }
 
-// gW[o][i] += sum_r dZ[r][o] X[r][i], gb[o] += sum_r dZ[r][o]: a block sums a
-// chunk of rows staged in LDS, then adds to the gradient with atomics
+// gW[o][i] += sum_r dZ[r][o] X[r * ldx + i], gb[o] += sum_r dZ[r][o] (gb
+// optional): a block sums a chunk of rows staged in LDS, then adds to the
+// gradient with atomics
 constexpr int kRowChunk = 64;
-__global__ __launch_bounds__(kT) void k_wgrad(long long rows, int K, int cout, const float* dZ, const float* X,
-                                              float* gW, float* gb) {
+__global__ __launch_bounds__(kT) void k_wgrad(long long rows, int K, int ldx, int cout, const float* dZ,
+                                              const float* X, float* gW, float* gb) {
   __shared__ float sz[kRowChunk * 32];
   __shared__ float sx[kRowChunk * 32];
   const long long r0 = (long long)blockIdx.x * kRowChunk;
   const int nr = (int)((rows - r0) < kRowChunk ? (rows - r0) : kRowChunk);
   for (int t = threadIdx.x; t < nr * cout; t += blockDim.x) sz[t] = dZ[r0 * cout + t];
-  for (int t = threadIdx.x; t < nr * K; t += blockDim.x) sx[t] = X[r0 * K + t];
+  for (int t = threadIdx.x; t < nr * K; t += blockDim.x) sx[t] = X[(r0 + t / K) * ldx + t % K];
   __syncthreads();
-  for (int p = threadIdx.x; p < cout * (K + 1); p += blockDim.x) {
-    const int o = p / (K + 1), i = p % (K + 1);
+  const int ncol = gb ? K + 1 : K;
+  for (int p = threadIdx.x; p < cout * ncol; p += blockDim.x) {
+    const int o = p / ncol, i = p % ncol;
     float acc = 0.f;
     if (i < K)
       for (int r = 0; r < nr; ++r) acc += sz[r * cout + o] * sx[r * K + i];
@@ -217,9 +217,13 @@ __global__ __launch_bounds__(kT) void k_wgrad(long long rows, int K, int cout, c
   }
 }
 
-__global__ __launch_bounds__(kT) void k_gelu_bwd(long long n, const float* z, float* d) {
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
-    d[t] *= gelu_grad(z[t]);
+// gate backward: dz = d * g (in place in d), dgp = d * z * g (1 - g)
+__global__ __launch_bounds__(kT) void k_gate_bwd(long long n, const float* g, const float* z, float* d, float* dgp) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const float gv = g[t], dv = d[t];
+    dgp[t] = dv * z[t] * gv * (1.f - gv);
+    d[t] = dv * gv;
+  }
 }
 
 __global__ __launch_bounds__(kT) void k_add(long long n, float* y, const float* x) {
@@ -327,9 +331,11 @@ __global__ void k_scale(int B, int nb, long long n, const double* part, double* 
   scale[b] = s > 0.0 && isfinite(s) ? s : 1.0;
 }
 
-// features of coarse cell (b, X, Y, Z): mean of r / scale over the f^3 fine
-// cells (2 C channels: re, im per component), then the precomputed chi
-// features (nchi channels, copied)
+// features of coarse cell (b, X, Y, Z) for the two real samples of RHS b
+// (sample 2b: real parts, 2b + 1: imaginary parts): the mean of the field /
+// scale over the f^3 fine cells (C channels), then the nchi material
+// features.  Running one real network on both samples makes the
+// preconditioner complex-linear.
 template <typename T>
 __global__ __launch_bounds__(kT) void k_restrict(int B, int C, int nx, int ny, int nz, int f, const cpx<T>* r,
                                                  const double* scale, const float* chi_feat, int nchi, float* feat,
@@ -339,7 +345,7 @@ __global__ __launch_bounds__(kT) void k_restrict(int B, int C, int nx, int ny, i
   const long long cells = (long long)gx * gy * gz;
   const long long total = (long long)B * cells;
   const long long M = (long long)nx * ny * nz;
-  const int nin = 2 * C + nchi;
+  const int nin = C + nchi;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     const int b = (int)(t / cells);
@@ -348,7 +354,8 @@ __global__ __launch_bounds__(kT) void k_restrict(int B, int C, int nx, int ny, i
     const int Y = (int)((cell / gz) % gy);
     const int X = (int)(cell / ((long long)gz * gy));
     const float inv = (float)(1.0 / (scale[b] * (double)f * f * f));
-    float* o = feat + t * nin;
+    float* ore = feat + ((size_t)(2 * b) * cells + cell) * nin;
+    float* oim = feat + ((size_t)(2 * b + 1) * cells + cell) * nin;
     for (int c = 0; c < C; ++c) {
       float re = 0.f, im = 0.f;
       const cpx<T>* rc = r + ((size_t)b * C + c) * M;
@@ -359,34 +366,116 @@ __global__ __launch_bounds__(kT) void k_restrict(int B, int C, int nx, int ny, i
             re += (float)v.re;
             im += (float)v.im;
           }
-      o[2 * c] = re * inv;
-      o[2 * c + 1] = im * inv;
+      ore[c] = re * inv;
+      oim[c] = im * inv;
     }
-    for (int k = 0; k < nchi; ++k) o[2 * C + k] = chi_feat[cell * nchi + k];
+    for (int k = 0; k < nchi; ++k) ore[C + k] = oim[C + k] = chi_feat[cell * nchi + k];
   }
 }
 
-// out = r + scale P(y): y [B][cells][2C] (re, im per component)
+// Trilinear prolongation P: the coarse value of cell X sits at fine
+// position (X + 1/2) f - 1/2; a fine cell takes the trilinear blend of the
+// 8 surrounding coarse cells (clamped at the faces, weights summing to 1).
+struct Lerp {
+  int i0, i1;
+  float w0, w1;
+};
+__device__ __forceinline__ Lerp lerp_axis(int x, int f, int G) {
+  const float xi = ((float)x + 0.5f) / (float)f - 0.5f;
+  int i0 = (int)floorf(xi);
+  float w1 = xi - (float)i0;
+  int i1 = i0 + 1;
+  if (i0 < 0) { i0 = 0; w1 = 0.f; }
+  if (i1 > G - 1) { i1 = G - 1; }
+  Lerp l;
+  l.i0 = i0; l.i1 = i1; l.w0 = 1.f - w1; l.w1 = w1;
+  return l;
+}
+
+// sum_corners w y[cell][c] for the real and imaginary samples (2b, 2b + 1)
+__device__ __forceinline__ void blend(const float* y, int b, long long cells, int C, int c, int gy, int gz, Lerp lx,
+                                      Lerp ly, Lerp lz, float* qre, float* qim) {
+  float re = 0.f, im = 0.f;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int X = (a & 4) ? lx.i1 : lx.i0, Y = (a & 2) ? ly.i1 : ly.i0, Z = (a & 1) ? lz.i1 : lz.i0;
+    const float w = ((a & 4) ? lx.w1 : lx.w0) * ((a & 2) ? ly.w1 : ly.w0) * ((a & 1) ? lz.w1 : lz.w0);
+    const long long cell = ((long long)X * gy + Y) * gz + Z;
+    re += w * y[((size_t)(2 * b) * cells + cell) * C + c];
+    im += w * y[((size_t)(2 * b + 1) * cells + cell) * C + c];
+  }
+  *qre = re;
+  *qim = im;
+}
+
+// out = base + scale P(y) (base may be null), y [2B][cells][C] (sample 2b:
+// real parts, 2b + 1: imaginary parts)
 template <typename T>
-__global__ __launch_bounds__(kT) void k_prolong_add(int B, int C, int nx, int ny, int nz, int f, const cpx<T>* r,
-                                                    const double* scale, const float* y, cpx<T>* out,
-                                                    const int* skip) {
+__global__ __launch_bounds__(kT) void k_prolong(int B, int C, int nx, int ny, int nz, int f, const cpx<T>* base,
+                                                const double* scale, const float* y, cpx<T>* out, const int* skip) {
   if (skip && *skip) return;
   const long long M = (long long)nx * ny * nz;
   const long long total = (long long)B * C * M;
-  const int gy = ny / f, gz = nz / f;
-  const long long cells = (long long)(nx / f) * gy * gz;
+  const int gx = nx / f, gy = ny / f, gz = nz / f;
+  const long long cells = (long long)gx * gy * gz;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     const long long i = t % M;
     const long long bc = t / M;
     const int c = (int)(bc % C), b = (int)(bc / C);
     const int z = (int)(i % nz), yy = (int)((i / nz) % ny), x = (int)(i / ((long long)nz * ny));
-    const long long cell = ((long long)(x / f) * gy + yy / f) * gz + z / f;
-    const float* q = y + ((size_t)b * cells + cell) * 2 * C + 2 * c;
+    float qre, qim;
+    blend(y, b, cells, C, c, gy, gz, lerp_axis(x, f, gx), lerp_axis(yy, f, gy), lerp_axis(z, f, gz), &qre, &qim);
     const double s = scale[b];
-    const cpx<T> v = r[t];
-    out[t] = mk((T)((double)v.re + s * q[0]), (T)((double)v.im + s * q[1]));
+    const cpx<T> v = base ? base[t] : mk((T)0, (T)0);
+    out[t] = mk((T)((double)v.re + s * qre), (T)((double)v.im + s * qim));
+  }
+}
+
+// adjoint of P: out[2b + p][cell][c] = (sum over fine cells x of the weight of
+// `cell` at x times Re / Im g_b(x)) / scale[b]; a coarse cell's support spans
+// its own f cells and half a cell's worth on each side
+template <typename T>
+__global__ __launch_bounds__(kT) void k_prolong_t(int B, int C, int nx, int ny, int nz, int f, const cpx<T>* g,
+                                                  const double* scale, float* out) {
+  const int gx = nx / f, gy = ny / f, gz = nz / f;
+  const long long cells = (long long)gx * gy * gz;
+  const long long M = (long long)nx * ny * nz;
+  const long long total = (long long)B * cells * C;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(t % C);
+    const long long q = t / C;
+    const long long cell = q % cells;
+    const int b = (int)(q / cells);
+    const int Z = (int)(cell % gz), Y = (int)((cell / gz) % gy), X = (int)(cell / ((long long)gz * gy));
+    const cpx<T>* gc = g + ((size_t)b * C + c) * M;
+    float re = 0.f, im = 0.f;
+    for (int x = (X - 1) * f; x < (X + 2) * f; ++x) {
+      if (x < 0 || x >= nx) continue;
+      const Lerp lx = lerp_axis(x, f, gx);
+      const float wx = (lx.i0 == X ? lx.w0 : 0.f) + (lx.i1 == X ? lx.w1 : 0.f);
+      if (wx == 0.f) continue;
+      for (int yy = (Y - 1) * f; yy < (Y + 2) * f; ++yy) {
+        if (yy < 0 || yy >= ny) continue;
+        const Lerp ly = lerp_axis(yy, f, gy);
+        const float wy = (ly.i0 == Y ? ly.w0 : 0.f) + (ly.i1 == Y ? ly.w1 : 0.f);
+        if (wy == 0.f) continue;
+        for (int z = (Z - 1) * f; z < (Z + 2) * f; ++z) {
+          if (z < 0 || z >= nz) continue;
+          const Lerp lz = lerp_axis(z, f, gz);
+          const float wz = (lz.i0 == Z ? lz.w0 : 0.f) + (lz.i1 == Z ? lz.w1 : 0.f);
+          if (wz == 0.f) continue;
+          const cpx<T> v = gc[((long long)x * ny + yy) * nz + z];
+          const float w = wx * wy * wz;
+          re += w * (float)v.re;
+          im += w * (float)v.im;
+        }
+      }
+    }
+    const float inv = (float)(1.0 / scale[b]);
+    out[((size_t)(2 * b) * cells + cell) * C + c] = re * inv;
+    out[((size_t)(2 * b + 1) * cells + cell) * C + c] = im * inv;
   }
 }
 
@@ -394,7 +483,7 @@ __global__ __launch_bounds__(kT) void k_prolong_add(int B, int C, int nx, int ny
 // of the 3 diagonal entries; 1 entry for the acoustic / isotropic case)
 template <typename T>
 __global__ __launch_bounds__(kT) void k_chi_feat(int nx, int ny, int nz, int f, const cpx<T>* chi, int chi_kind,
-                                                 int nchi, float* out) {
+                                                 int nchi, float extra, float* out) {
   const int gx = nx / f, gy = ny / f, gz = nz / f;
   const long long cells = (long long)gx * gy * gz;
   const long long M = (long long)nx * ny * nz;
@@ -421,6 +510,7 @@ __global__ __launch_bounds__(kT) void k_chi_feat(int nx, int ny, int nz, int f, 
       out[cell * nchi + 2 * a] = re * inv;
       out[cell * nchi + 2 * a + 1] = im * inv;
     }
+    if (nchi & 1) out[cell * nchi + nchi - 1] = extra;
   }
 }
 
@@ -434,38 +524,39 @@ namespace e3 {
 std::string fno_check(const em3d_fno_config* c) {
   if (!c) return "null config";
   if (c->grid < 2 || c->channels < 1 || c->channels > 32 || c->modes < 2 || (c->modes & 1) || c->modes > c->grid ||
-      c->layers < 0 || c->layers > 16 || c->in_channels < 1 || c->in_channels > 32 || c->out_channels < 1 ||
-      c->out_channels > 32)
-    return "invalid FNO config (grid >= 2, 1 <= channels/in/out <= 32, even modes <= grid, layers <= 16)";
+      c->layers < 0 || c->layers > kFnoMaxLayers || c->in_channels < 1 || c->in_channels > 32 ||
+      c->out_channels < 1 || c->out_channels > 32 || c->material_channels < 0 || c->material_channels > 32)
+    return "invalid FNO config (grid >= 2, 1 <= channels/in/out <= 32, material <= 32, even modes <= grid, "
+           "layers <= 16)";
   return "";
 }
 
 FnoLayout fno_layout(const em3d_fno_config& c, int S) {
   FnoLayout L{};
-  const long long G = c.grid, H = c.channels, M = c.modes;
+  const long long G = c.grid, H = c.channels, M = c.modes, nm = c.material_channels;
   L.cells = G * G * G;
   L.modes3 = M * M * M;
+  L.ldx = c.in_channels + c.material_channels;
   long long o = 0;
   L.w_in = o; o += H * c.in_channels;
-  L.b_in = o; o += H;
   for (int l = 0; l < c.layers; ++l) {
     L.w[l] = o; o += H * H;
-    L.b[l] = o; o += H;
+    L.u[l] = o; o += H * nm;
+    L.d[l] = o; o += H;
     L.r[l] = o; o += L.modes3 * H * H * 2;
   }
   L.w_out = o; o += (long long)c.out_channels * H;
-  L.b_out = o; o += c.out_channels;
   L.n_params = o;
-  // work (floats): x-independent activations per sample
+  // work (floats)
   long long w = 0;
   auto take = [&](long long n) { const long long at = w; w += (n + 63) / 64 * 64; return at; };
   const long long ch = L.cells * H;
   for (int l = 0; l <= c.layers; ++l) L.act[l] = take(S * ch);
   for (int l = 0; l < c.layers; ++l) L.zpre[l] = take(S * ch);
+  for (int l = 0; l < c.layers; ++l) L.gate[l] = take(S * ch);
   L.a1 = take(S * G * G * M * H * 2);
   L.a2 = take(S * G * M * M * H * 2);
-  L.a3[0] = take(S * L.modes3 * H * 2);
-  for (int l = 1; l < c.layers; ++l) L.a3[l] = take(S * L.modes3 * H * 2);
+  for (int l = 0; l < c.layers; ++l) L.a3[l] = take(S * L.modes3 * H * 2);
   L.b3 = take(S * L.modes3 * H * 2);
   L.spec = take(S * ch);
   L.d0 = take(S * ch);
@@ -494,10 +585,17 @@ struct Ctx {
   const float2* tw() const { return (const float2*)(w + L.tw); }
 };
 
-void mix(Ctx& x, long long rows, int K, int cout, const float* A, const unsigned short* W, int transpose,
-         const float* bias, const float* add, float* Z, float* Y, int act) {
+void mix(Ctx& x, long long rows, int K, int lda, int cout, const float* A, const unsigned short* W, int transpose,
+         const float* bias, const float* add, const float* gate, float* Z, float* Y, int act) {
   const unsigned nb = (unsigned)((rows + 63) / 64);
-  hipLaunchKernelGGL(k_mix, dim3(nb), dim3(kT), 0, x.st, rows, K, cout, A, W, transpose, bias, add, Z, Y, act, x.skip);
+  hipLaunchKernelGGL(k_mix, dim3(nb), dim3(kT), 0, x.st, rows, K, lda, cout, A, W, transpose, bias, add, gate, Z, Y,
+                     act, x.skip);
+  x.note();
+}
+
+void wgrad(Ctx& x, long long rows, int K, int ldx, int cout, const float* dZ, const float* X, float* gW, float* gb) {
+  const unsigned nb = (unsigned)((rows + kRowChunk - 1) / kRowChunk);
+  hipLaunchKernelGGL(k_wgrad, dim3(nb), dim3(kT), 0, x.st, rows, K, ldx, cout, dZ, X, gW, gb);
   x.note();
 }
 
@@ -536,14 +634,17 @@ int fno_forward_impl(const em3d_fno_config& c, const FnoLayout& L, const unsigne
                      const float* xin, float* y, float* work, int S, hipStream_t st, const int* skip) {
   Ctx x{c, L, work, S, st, skip};
   const long long rows = (long long)S * L.cells;
-  const int H = c.channels;
-  mix(x, rows, c.in_channels, H, xin, pb + L.w_in, 0, params + L.b_in, nullptr, nullptr, x.at(L.act[0]), 0);
+  const int H = c.channels, ldx = (int)L.ldx, nm = c.material_channels;
+  mix(x, rows, c.in_channels, ldx, H, xin, pb + L.w_in, 0, nullptr, nullptr, nullptr, nullptr, x.at(L.act[0]), 0);
   for (int l = 0; l < c.layers; ++l) {
+    // gates from the material channels (zero channels: sigmoid(d_l))
+    mix(x, rows, nm, ldx, H, xin + c.in_channels, pb + L.u[l], 0, params + L.d[l], nullptr, nullptr, nullptr,
+        x.at(L.gate[l]), 2);
     spectral(x, l, x.at(L.act[l]), pb + L.r[l], x.at(L.spec), x.at(L.a3[l]));
-    mix(x, rows, H, H, x.at(L.act[l]), pb + L.w[l], 0, params + L.b[l], x.at(L.spec), x.at(L.zpre[l]),
+    mix(x, rows, H, H, H, x.at(L.act[l]), pb + L.w[l], 0, nullptr, x.at(L.spec), x.at(L.gate[l]), x.at(L.zpre[l]),
         x.at(L.act[l + 1]), 1);
   }
-  mix(x, rows, H, c.out_channels, x.at(L.act[c.layers]), pb + L.w_out, 0, params + L.b_out, nullptr, nullptr, y, 0);
+  mix(x, rows, H, H, c.out_channels, x.at(L.act[c.layers]), pb + L.w_out, 0, nullptr, nullptr, nullptr, nullptr, y, 0);
   return x.err == hipSuccess ? 0 : set_error(std::string("em3d_fno_forward: ") + hipGetErrorString(x.err));
 }
 
@@ -551,24 +652,23 @@ int fno_backward_impl(const em3d_fno_config& c, const FnoLayout& L, const unsign
                       const float* dy, float* grad, float* work, int S, hipStream_t st) {
   Ctx x{c, L, work, S, st, nullptr};
   const long long rows = (long long)S * L.cells;
-  const int H = c.channels, G = c.grid, M = c.modes;
-  const unsigned nbw = (unsigned)((rows + kRowChunk - 1) / kRowChunk);
-  // output layer
-  hipLaunchKernelGGL(k_wgrad, dim3(nbw), dim3(kT), 0, st, rows, H, c.out_channels, dy, x.at(L.act[c.layers]),
-                     grad + L.w_out, grad + L.b_out);
-  x.note();
+  const int H = c.channels, G = c.grid, M = c.modes, ldx = (int)L.ldx, nm = c.material_channels;
+  wgrad(x, rows, H, H, c.out_channels, dy, x.at(L.act[c.layers]), grad + L.w_out, nullptr);
   float* d = x.at(L.d0);   // dL/d act[l]
   float* e = x.at(L.d1);
-  mix(x, rows, c.out_channels, H, dy, pb + L.w_out, 1, nullptr, nullptr, nullptr, d, 0);
+  mix(x, rows, c.out_channels, c.out_channels, H, dy, pb + L.w_out, 1, nullptr, nullptr, nullptr, nullptr, d, 0);
   float* a1 = x.at(L.a1);
   float* a2 = x.at(L.a2);
   float* b3 = x.at(L.b3);
+  float* dgp = x.at(L.spec);
   for (int l = c.layers - 1; l >= 0; --l) {
-    // dz = d * gelu'(z)
-    hipLaunchKernelGGL(k_gelu_bwd, dim3(grid1(rows * H)), dim3(kT), 0, st, rows * H, x.at(L.zpre[l]), d);
+    // d <- dz = d * g, dgp = d * z * g (1 - g)
+    hipLaunchKernelGGL(k_gate_bwd, dim3(grid1(rows * H)), dim3(kT), 0, st, rows * H, x.at(L.gate[l]),
+                       x.at(L.zpre[l]), d, dgp);
     x.note();
-    hipLaunchKernelGGL(k_wgrad, dim3(nbw), dim3(kT), 0, st, rows, H, H, d, x.at(L.act[l]), grad + L.w[l], grad + L.b[l]);
-    x.note();
+    if (nm > 0) wgrad(x, rows, nm, ldx, H, dgp, xin + c.in_channels, grad + L.u[l], grad + L.d[l]);
+    else wgrad(x, rows, 0, ldx, H, dgp, xin, grad + L.u[l], grad + L.d[l]);
+    wgrad(x, rows, H, H, H, d, x.at(L.act[l]), grad + L.w[l], nullptr);
     // spectral adjoint: gB = E-(dz) / G^3, gR += gB conj(A), gA = R^H gB, e = Re(E+ gA)
     dft(x, (long long)S * G * G, G, M, H, 0, -1, d, 1, a1, 0, 1.f / ((float)G * G * G));
     dft(x, (long long)S * G, G, M, M * H, 0, -1, a1, 0, a2, 0, 1.f);
@@ -584,13 +684,12 @@ int fno_backward_impl(const em3d_fno_config& c, const FnoLayout& L, const unsign
     dft(x, (long long)S * G, M, G, M * H, 1, 1, a2, 0, a1, 0, 1.f);
     dft(x, (long long)S * G * G, M, G, H, 1, 1, a1, 0, e, 1, 1.f);
     // e += W^T dz  (the pointwise path), then e is dL/d act[l]
-    mix(x, rows, H, H, d, pb + L.w[l], 1, nullptr, e, nullptr, e, 0);
+    mix(x, rows, H, H, H, d, pb + L.w[l], 1, nullptr, e, nullptr, nullptr, e, 0);
     float* t = d;
     d = e;
     e = t;
   }
-  hipLaunchKernelGGL(k_wgrad, dim3(nbw), dim3(kT), 0, st, rows, c.in_channels, H, d, xin, grad + L.w_in, grad + L.b_in);
-  x.note();
+  wgrad(x, rows, c.in_channels, ldx, H, d, xin, grad + L.w_in, nullptr);
   return x.err == hipSuccess ? 0 : set_error(std::string("em3d_fno_backward: ") + hipGetErrorString(x.err));
 }
 
@@ -609,7 +708,7 @@ int fno_twiddles(const em3d_fno_config& c, const FnoLayout& L, float* work, hipS
 int fno_precond_apply(const em3d_fno_model* m, int B, int C, int nx, int ny, int nz, int prec, const void* r, void* out,
                       const int* skip, hipStream_t st) {
   const em3d_fno_config& c = m->config;
-  const FnoLayout L = fno_layout(c, B);
+  const FnoLayout L = fno_layout(c, 2 * B);   // two real samples per right-hand side
   float* w = (float*)m->work;
   const long long n = (long long)C * nx * ny * nz;
   const unsigned nb = 256;
@@ -631,13 +730,13 @@ int fno_precond_apply(const em3d_fno_model* m, int B, int C, int nx, int ny, int
                        (const cpx<float>*)r, scale, m->chi_features, m->chi_channels, feat, skip);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(std::string("fno precond: ") + hipGetErrorString(e));
-  float* y = feat + B * cells * c.in_channels;
-  if (fno_forward_impl(c, L, m->params_bf16, m->params, feat, y, w, B, st, skip)) return 1;
+  float* y = feat + 2 * B * cells * L.ldx;
+  if (fno_forward_impl(c, L, m->params_bf16, m->params, feat, y, w, 2 * B, st, skip)) return 1;
   if (prec == 1)
-    hipLaunchKernelGGL(k_prolong_add<double>, dim3(grid1(B * n)), dim3(kT), 0, st, B, C, nx, ny, nz, f,
+    hipLaunchKernelGGL(k_prolong<double>, dim3(grid1(B * n)), dim3(kT), 0, st, B, C, nx, ny, nz, f,
                        (const cpx<double>*)r, scale, y, (cpx<double>*)out, skip);
   else
-    hipLaunchKernelGGL(k_prolong_add<float>, dim3(grid1(B * n)), dim3(kT), 0, st, B, C, nx, ny, nz, f,
+    hipLaunchKernelGGL(k_prolong<float>, dim3(grid1(B * n)), dim3(kT), 0, st, B, C, nx, ny, nz, f,
                        (const cpx<float>*)r, scale, y, (cpx<float>*)out, skip);
   e = hipGetLastError();
   return e == hipSuccess ? 0 : set_error(std::string("fno precond: ") + hipGetErrorString(e));
@@ -719,16 +818,17 @@ int em3d_fno_to_bf16(long long n, const float* params, unsigned short* params_bf
 }
 
 int em3d_fno_chi_features(int nx, int ny, int nz, int factor, int precision, const void* chi, int chi_kind,
-                          int chi_channels, float* out, hipStream_t stream) {
-  if (!out || factor < 1 || nx % factor || ny % factor || nz % factor || (chi_channels != 2 && chi_channels != 6))
-    return set_error("em3d_fno_chi_features: bad argument (factor must divide the grid; 2 or 6 channels)");
+                          int chi_channels, double extra, float* out, hipStream_t stream) {
+  if (!out || factor < 1 || nx % factor || ny % factor || nz % factor ||
+      (chi_channels != 2 && chi_channels != 3 && chi_channels != 6 && chi_channels != 7))
+    return set_error("em3d_fno_chi_features: bad argument (factor must divide the grid; 2, 3, 6 or 7 channels)");
   const long long cells = (long long)(nx / factor) * (ny / factor) * (nz / factor);
   if (precision == 1)
     hipLaunchKernelGGL(k_chi_feat<double>, dim3(grid1(cells)), dim3(kT), 0, stream, nx, ny, nz, factor,
-                       (const cpx<double>*)chi, chi_kind, chi_channels, out);
+                       (const cpx<double>*)chi, chi_kind, chi_channels, (float)extra, out);
   else
     hipLaunchKernelGGL(k_chi_feat<float>, dim3(grid1(cells)), dim3(kT), 0, stream, nx, ny, nz, factor,
-                       (const cpx<float>*)chi, chi_kind, chi_channels, out);
+                       (const cpx<float>*)chi, chi_kind, chi_channels, (float)extra, out);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_error(std::string("em3d_fno_chi_features: ") + hipGetErrorString(e));
 }
@@ -736,7 +836,7 @@ int em3d_fno_chi_features(int nx, int ny, int nz, int factor, int precision, con
 size_t em3d_fno_features_bytes(const em3d_fno_config* c, int samples) {
   if (!fno_check(c).empty() || samples < 1) return 0;
   const long long cells = (long long)c->grid * c->grid * c->grid;
-  return (size_t)samples * cells * (c->in_channels + c->out_channels) * sizeof(float);
+  return (size_t)samples * cells * (c->in_channels + c->material_channels + c->out_channels) * sizeof(float);
 }
 
 size_t em3d_fno_scale_work_bytes(int batch) { return (size_t)batch * 256 * sizeof(double); }
@@ -774,6 +874,36 @@ int em3d_fno_restrict(int batch, int ncomp, int nx, int ny, int nz, int factor, 
   return e == hipSuccess ? 0 : set_error(std::string("em3d_fno_restrict: ") + hipGetErrorString(e));
 }
 
+int em3d_fno_prolong(int batch, int ncomp, int nx, int ny, int nz, int factor, int precision, const void* base,
+                     const double* scale, const float* y, void* out, hipStream_t stream) {
+  if (!scale || !y || !out || batch < 1 || factor < 1 || nx % factor || ny % factor || nz % factor)
+    return set_error("em3d_fno_prolong: bad argument (factor must divide the grid)");
+  const long long n = (long long)batch * ncomp * nx * ny * nz;
+  if (precision == 1)
+    hipLaunchKernelGGL(k_prolong<double>, dim3(grid1(n)), dim3(kT), 0, stream, batch, ncomp, nx, ny, nz, factor,
+                       (const cpx<double>*)base, scale, y, (cpx<double>*)out, (const int*)nullptr);
+  else
+    hipLaunchKernelGGL(k_prolong<float>, dim3(grid1(n)), dim3(kT), 0, stream, batch, ncomp, nx, ny, nz, factor,
+                       (const cpx<float>*)base, scale, y, (cpx<float>*)out, (const int*)nullptr);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_error(std::string("em3d_fno_prolong: ") + hipGetErrorString(e));
+}
+
+int em3d_fno_prolong_adjoint(int batch, int ncomp, int nx, int ny, int nz, int factor, int precision, const void* g,
+                             const double* scale, float* out, hipStream_t stream) {
+  if (!g || !scale || !out || batch < 1 || factor < 1 || nx % factor || ny % factor || nz % factor)
+    return set_error("em3d_fno_prolong_adjoint: bad argument (factor must divide the grid)");
+  const long long n = (long long)batch * ncomp * (nx / factor) * (ny / factor) * (nz / factor);
+  if (precision == 1)
+    hipLaunchKernelGGL(k_prolong_t<double>, dim3(grid1(n)), dim3(kT), 0, stream, batch, ncomp, nx, ny, nz, factor,
+                       (const cpx<double>*)g, scale, out);
+  else
+    hipLaunchKernelGGL(k_prolong_t<float>, dim3(grid1(n)), dim3(kT), 0, stream, batch, ncomp, nx, ny, nz, factor,
+                       (const cpx<float>*)g, scale, out);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : set_error(std::string("em3d_fno_prolong_adjoint: ") + hipGetErrorString(e));
+}
+
 int em3d_fno_precondition(const em3d_fno_model* model, int batch, int ncomp, int nx, int ny, int nz, int precision,
                           const void* r, void* out, hipStream_t stream) {
   if (!model || !r || !out || batch < 1) return set_error("em3d_fno_precondition: null argument");
@@ -782,8 +912,10 @@ int em3d_fno_precondition(const em3d_fno_model* model, int batch, int ncomp, int
   const int f = model->factor;
   if (f < 1 || nx != model->config.grid * f || ny != model->config.grid * f || nz != model->config.grid * f)
     return set_error("em3d_fno_precondition: the grid must be (config.grid * factor)^3");
-  if (model->config.in_channels != 2 * ncomp + model->chi_channels || model->config.out_channels != 2 * ncomp)
-    return set_error("em3d_fno_precondition: in_channels must be 2 ncomp + chi_channels, out_channels 2 ncomp");
+  if (model->config.in_channels != ncomp || model->config.material_channels != model->chi_channels ||
+      model->config.out_channels != ncomp)
+    return set_error("em3d_fno_precondition: in_channels and out_channels must be ncomp, material_channels "
+                     "chi_channels");
   return fno_precond_apply(model, batch, ncomp, nx, ny, nz, precision, r, out, nullptr, stream);
 }
 

@@ -14,8 +14,10 @@ constexpr int kFnoMaxLayers = 16;
 // offsets (in floats) of the parameters and of the work buffer
 struct FnoLayout {
   long long cells, modes3;
-  long long w_in, b_in, w[kFnoMaxLayers], b[kFnoMaxLayers], r[kFnoMaxLayers], w_out, b_out, n_params;
-  long long act[kFnoMaxLayers + 1], zpre[kFnoMaxLayers], a1, a2, a3[kFnoMaxLayers], b3, spec, d0, d1, part, tw;
+  long long ldx;   // row stride of the input features: residual then material channels
+  long long w_in, w[kFnoMaxLayers], u[kFnoMaxLayers], d[kFnoMaxLayers], r[kFnoMaxLayers], w_out, n_params;
+  long long act[kFnoMaxLayers + 1], zpre[kFnoMaxLayers], gate[kFnoMaxLayers], a1, a2, a3[kFnoMaxLayers], b3, spec,
+      d0, d1, part, tw;
   long long work_floats;
 };
 

@@ -628,8 +628,9 @@ int em3d_bicgstab_pc(em3d_plan_t plan, const void* khat, const void* chi, int ch
   if (!plan || precond->factor < 1 || P.N[0] != precond->config.grid * precond->factor ||
       P.N[1] != precond->config.grid * precond->factor || P.N[2] != precond->config.grid * precond->factor)
     return sfail("em3d_bicgstab_pc: the plan grid must be (config.grid * factor)^3");
-  if (precond->config.in_channels != 2 * P.C + precond->chi_channels || precond->config.out_channels != 2 * P.C)
-    return sfail("em3d_bicgstab_pc: in_channels must be 2 ncomp + chi_channels, out_channels 2 ncomp");
+  if (precond->config.in_channels != P.C || precond->config.material_channels != precond->chi_channels ||
+      precond->config.out_channels != P.C)
+    return sfail("em3d_bicgstab_pc: in_channels and out_channels must be ncomp, material_channels chi_channels");
   return bicgstab_impl(plan, khat, chi, chi_kind, rhs, u, work, solver_work, prm, history, action_counts, history_cap,
                        res, precond, stream);
 }

@@ -8,9 +8,11 @@ the (residual, correction) training pairs from the library's own solves and
 wraps the trained network as the preconditioner of ``FlexibleBiCGStab``.
 
 Preconditioner:  M r = r + s P(FNO(R r / s, chi)),  s = |r| / sqrt(n), where R
-averages factor^3 fine cells into one coarse cell and P copies back.  The
-network is trained so that M r approximates A^-1 r on the coarse grid:
-target = R(A^-1 r - r) / s.
+averages factor^3 fine cells into one coarse cell and P interpolates back
+trilinearly.  The
+network is trained to minimise the preconditioned residual
+|A M r - r|^2 / |r|^2 over residuals of the library's own solves
+(``em3d_amd.learn``).
 """
 from __future__ import annotations
 
@@ -32,19 +34,24 @@ class FNOConfig:
     layers: int = 4
 
 
-def _cfg(c: FNOConfig, n_in: int, n_out: int):
-    return _lib.FnoConfig(c.grid, c.channels, c.modes, c.layers, n_in, n_out)
+def _cfg(c: FNOConfig, n_in: int, n_out: int, n_mat: int):
+    return _lib.FnoConfig(c.grid, c.channels, c.modes, c.layers, n_in, n_out, n_mat)
 
 
 class FNO:
-    """Parameters (fp32 master + bf16 copy), gradients and Adam state on the device."""
+    """Parameters (fp32 master + bf16 copy), gradients and Adam state on the device.
 
-    def __init__(self, config: FNOConfig, in_channels: int, out_channels: int, *, device, seed: int = 0):
+    The network is linear in its ``in_channels`` inputs (the residual) and
+    conditioned on ``material_channels`` inputs through multiplicative gates."""
+
+    def __init__(self, config: FNOConfig, in_channels: int, out_channels: int, *, material_channels: int = 0,
+                 device, seed: int = 0):
         import torch
         self.config = config
         self.in_channels, self.out_channels = int(in_channels), int(out_channels)
+        self.material_channels = int(material_channels)
         self.device = device
-        self.cstruct = _cfg(config, self.in_channels, self.out_channels)
+        self.cstruct = _cfg(config, self.in_channels, self.out_channels, self.material_channels)
         lib = _lib.lib()
         n = int(lib.em3d_fno_param_count(C.byref(self.cstruct)))
         if n <= 0:
@@ -59,14 +66,15 @@ class FNO:
         self._work = {}
         self.refresh_bf16()
 
-    # parameter layout of e3_fno.hip: W_in, b_in, per layer (W, b, R), W_out, b_out
+    # parameter layout of e3_fno.hip: W_in, per layer (W, U, d, R), W_out
     def _sizes(self):
         c, H = self.config, self.config.channels
         m3 = c.modes ** 3
-        out = [("w_in", H * self.in_channels), ("b_in", H)]
+        out = [("w_in", H * self.in_channels)]
         for l in range(c.layers):
-            out += [(f"w{l}", H * H), (f"b{l}", H), (f"r{l}", m3 * H * H * 2)]
-        out += [("w_out", self.out_channels * H), ("b_out", self.out_channels)]
+            out += [(f"w{l}", H * H), (f"u{l}", H * self.material_channels), (f"d{l}", H),
+                    (f"r{l}", m3 * H * H * 2)]
+        out += [("w_out", self.out_channels * H)]
         return out
 
     def _init(self, seed: int) -> np.ndarray:
@@ -76,11 +84,13 @@ class FNO:
         for name, size in self._sizes():
             if name == "w_in":
                 parts.append(rng.standard_normal(size) / np.sqrt(self.in_channels))
+            elif name.startswith("u"):
+                parts.append(rng.standard_normal(size) / np.sqrt(max(1, self.material_channels)))
             elif name.startswith("w") and name != "w_out":
                 parts.append(rng.standard_normal(size) / np.sqrt(H))
             elif name.startswith("r"):
                 parts.append(rng.standard_normal(size) * (0.5 / H))
-            else:   # biases, and W_out = 0: the preconditioner starts as the identity
+            else:   # gate biases, and W_out = 0: the preconditioner starts as the identity
                 parts.append(np.zeros(size))
         p = np.concatenate(parts).astype(np.float32)
         assert p.size == self.n_params
@@ -134,18 +144,22 @@ class FNO:
                    "em3d_fno_loss")
         return loss / samples, dy
 
-    def train_step(self, x, target, *, lr: float = 1e-3, weight_decay: float = 0.0):
-        """One Adam step on a minibatch; returns the device loss before the step."""
-        samples = x.shape[0]
-        self.grad.zero_()
-        y = self.forward(x, samples)
-        loss, dy = self.loss_and_grad(y, target, samples)
-        self.backward(x, dy, samples)
+    def adam_step(self, *, lr: float, weight_decay: float = 0.0):
+        """Apply the accumulated gradient (Adam) and refresh the bf16 copy."""
         self.step += 1
         _lib.check(_lib.lib().em3d_fno_adam(self.n_params, self.params.data_ptr(), self.params_bf16.data_ptr(),
                                             self.grad.data_ptr(), self.m.data_ptr(), self.v.data_ptr(), lr, 0.9,
                                             0.999, 1e-8, weight_decay, self.step, _lib.stream_handle(self.device)),
                    "em3d_fno_adam")
+
+    def train_step(self, x, target, *, lr: float = 1e-3, weight_decay: float = 0.0):
+        """One Adam step on a minibatch (target loss); returns the device loss before the step."""
+        samples = x.shape[0]
+        self.grad.zero_()
+        y = self.forward(x, samples)
+        loss, dy = self.loss_and_grad(y, target, samples)
+        self.backward(x, dy, samples)
+        self.adam_step(lr=lr, weight_decay=weight_decay)
         return loss
 
 
@@ -162,28 +176,32 @@ class FNOPreconditioner:
         self.operator = operator
         self.factor = N[0] // c.grid
         self.ncomp = operator._ncomp
-        self.chi_channels = 6 if self.ncomp == 3 else 2
-        if fno.in_channels != 2 * self.ncomp + self.chi_channels or fno.out_channels != 2 * self.ncomp:
-            raise ValueError("FNO channels do not match the operator (in 2 ncomp + chi, out 2 ncomp)")
+        # material channels: the diagonal of chi (re, im), plus k0 / 10 when the network takes it
+        self.chi_channels = fno.material_channels
+        if (self.chi_channels not in ((6, 7) if self.ncomp == 3 else (2, 3)) or fno.in_channels != self.ncomp
+                or fno.out_channels != self.ncomp):
+            raise ValueError("FNO channels do not match the operator (in = out = ncomp, material = chi [+ k0])")
         be = operator.backend
         dev = be.torch_device
         _, kind, chi = operator._spectrum()
         self.chi_features = torch.empty((fno.cells, self.chi_channels), dtype=torch.float32, device=dev)
         _lib.check(_lib.lib().em3d_fno_chi_features(*N, self.factor, be.precision.code,
                                                     chi.data_ptr() if chi is not None else None, kind,
-                                                    self.chi_channels, self.chi_features.data_ptr(),
-                                                    _lib.stream_handle(dev)), "em3d_fno_chi_features")
+                                                    self.chi_channels, float(operator._k0) / 10.0,
+                                                    self.chi_features.data_ptr(), _lib.stream_handle(dev)),
+                   "em3d_fno_chi_features")
         self._models = {}
 
     def model(self, batch: int):
-        """ctypes em3d_fno_model with work and feature buffers for `batch` RHS."""
+        """ctypes em3d_fno_model with work and feature buffers for `batch` RHS
+        (two real network samples each: real and imaginary parts)."""
         import torch
         m = self._models.get(batch)
         if m is None:
             f = self.fno
-            feats = torch.empty(int(_lib.lib().em3d_fno_features_bytes(C.byref(f.cstruct), batch)),
+            feats = torch.empty(int(_lib.lib().em3d_fno_features_bytes(C.byref(f.cstruct), 2 * batch)),
                                 dtype=torch.uint8, device=f.device)
-            w = f.work(batch)
+            w = f.work(2 * batch)
             st = _lib.FnoModel(f.cstruct, self.factor, f.params.data_ptr(), f.params_bf16.data_ptr(), w.data_ptr(),
                                feats.data_ptr(), self.chi_features.data_ptr(), self.chi_channels)
             m = (st, feats)
@@ -202,10 +220,60 @@ class FNOPreconditioner:
                    "em3d_fno_precondition")
         return out
 
+    def features(self, r):
+        """(features [2 batch][cells][in + material], scale s [batch]) of
+        residuals r: sample 2b holds the real parts of r_b, 2b + 1 the
+        imaginary parts."""
+        import torch
+        lib = _lib.lib()
+        op = self.operator
+        rb, batch = op._prepare_input(r)
+        dev = rb.device
+        n = rb.numel() // batch
+        scale = torch.empty(batch, dtype=torch.float64, device=dev)
+        sw = torch.empty(int(lib.em3d_fno_scale_work_bytes(batch)), dtype=torch.uint8, device=dev)
+        st = _lib.stream_handle(dev)
+        prec = op.backend.precision.code
+        _lib.check(lib.em3d_fno_scale(batch, n, prec, rb.data_ptr(), scale.data_ptr(), sw.data_ptr(), st),
+                   "em3d_fno_scale")
+        x = torch.empty((2 * batch, self.fno.cells, self.ncomp + self.chi_channels), dtype=torch.float32,
+                        device=dev)
+        _lib.check(lib.em3d_fno_restrict(batch, self.ncomp, *op._N, self.factor, prec, rb.data_ptr(),
+                                         scale.data_ptr(), self.chi_features.data_ptr(), self.chi_channels,
+                                         x.data_ptr(), st), "em3d_fno_restrict")
+        return x, scale
+
+    def prolong(self, y, scale, base=None):
+        """base + scale P(y_re + i y_im) on the operator's grid (y: [2 batch][cells][ncomp])."""
+        import torch
+        op = self.operator
+        batch = y.shape[0] // 2
+        shape = ((batch,) if batch > 1 or base is None else ()) + op._field_shape()
+        out = torch.empty(shape if base is None else base.shape, dtype=op.backend.precision.torch_complex_dtype,
+                          device=y.device)
+        _lib.check(_lib.lib().em3d_fno_prolong(batch, self.ncomp, *op._N, self.factor, op.backend.precision.code,
+                                               base.data_ptr() if base is not None else None, scale.data_ptr(),
+                                               y.data_ptr(), out.data_ptr(), _lib.stream_handle(y.device)),
+                   "em3d_fno_prolong")
+        return out
+
+    def prolong_adjoint(self, g, scale):
+        """[2 batch][cells][ncomp]: P^T of the real / imaginary parts of g, / scale."""
+        import torch
+        op = self.operator
+        gb, batch = op._prepare_input(g)
+        out = torch.empty((2 * batch, self.fno.cells, self.ncomp), dtype=torch.float32, device=gb.device)
+        _lib.check(_lib.lib().em3d_fno_prolong_adjoint(batch, self.ncomp, *op._N, self.factor,
+                                                       op.backend.precision.code, gb.data_ptr(), scale.data_ptr(),
+                                                       out.data_ptr(), _lib.stream_handle(gb.device)),
+                   "em3d_fno_prolong_adjoint")
+        return out
+
     # ---- training pairs -----------------------------------------------------
     def restrict_pair(self, r, e):
         """(features of r, target R(e - r)/s) for residuals r and corrections e
-        (A e = r), both (batch,) + field shape; returns fp32 [batch][cells][.]."""
+        (A e = r), both (batch,) + field shape; returns fp32 [2 batch][cells][.]
+        (real and imaginary samples)."""
         import torch
         lib = _lib.lib()
         op = self.operator
@@ -223,13 +291,13 @@ class FNOPreconditioner:
         _lib.check(lib.em3d_fno_scale(batch, n, prec, rb.data_ptr(), scale.data_ptr(), sw.data_ptr(), s),
                    "em3d_fno_scale")
         cells = self.fno.cells
-        x = torch.empty((batch, cells, 2 * self.ncomp + self.chi_channels), dtype=torch.float32, device=dev)
+        x = torch.empty((2 * batch, cells, self.ncomp + self.chi_channels), dtype=torch.float32, device=dev)
         _lib.check(lib.em3d_fno_restrict(batch, self.ncomp, *N, self.factor, prec, rb.data_ptr(), scale.data_ptr(),
                                          self.chi_features.data_ptr(), self.chi_channels, x.data_ptr(), s),
                    "em3d_fno_restrict")
         from .solvers._vec import lincomb
         d = lincomb(1.0, eb, -1.0, rb)
-        t = torch.empty((batch, cells, 2 * self.ncomp), dtype=torch.float32, device=dev)
+        t = torch.empty((2 * batch, cells, self.ncomp), dtype=torch.float32, device=dev)
         _lib.check(lib.em3d_fno_restrict(batch, self.ncomp, *N, self.factor, prec, d.data_ptr(), scale.data_ptr(),
                                          None, 0, t.data_ptr(), s), "em3d_fno_restrict")
         return x, t

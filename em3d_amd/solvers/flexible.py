@@ -34,14 +34,19 @@ def attach(preconditioner, operator):
 
 
 class FlexibleBiCGStab:
-    def __init__(self, config: SolverConfig, preconditioner, *, fallback: bool = True):
+    def __init__(self, config: SolverConfig, preconditioner, *, fallback: bool = True,
+                 preconditioned_max_iter: int | None = None):
         self.cfg = config
         self.preconditioner = preconditioner
         self.fallback = fallback
+        self.preconditioned_max_iter = preconditioned_max_iter   # budget of the preconditioned attempt
 
     def solve(self, operator, rhs) -> SolverResult:
         pc = attach(self.preconditioner, operator)
-        res = operator.device_bicgstab(rhs, self.cfg, precond=pc)
+        cfg = self.cfg
+        if self.preconditioned_max_iter is not None:
+            cfg = SolverConfig(**{**cfg.__dict__, "max_iter": min(cfg.max_iter, self.preconditioned_max_iter)})
+        res = operator.device_bicgstab(rhs, cfg, precond=pc)
         res.extras = {"preconditioner": "fno", "fallback": None}
         if res.status == "converged" or not self.fallback:
             return res

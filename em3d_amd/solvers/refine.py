@@ -15,7 +15,9 @@ experiments/stationary_validation.py:172-180) and the update ``x += d``:
     until |r| / |f| < max(rtol, atol / |f|)
 
 With a ``preconditioner`` (a trained FNO) the inner solves are the
-right-preconditioned device BiCGStab of ``FlexibleBiCGStab``.
+right-preconditioned device BiCGStab of ``FlexibleBiCGStab``; an inner solve
+that does not converge within ``preconditioned_max_iter`` iterations is
+repeated without the preconditioner.
 
 Each outer step divides the residual by about ``inner_rtol``, so rtol 1e-8
 takes two or three steps.  When a step fails to reduce the true residual by
@@ -60,6 +62,7 @@ class MixedBiCGStab:
         self.stagnation = float(stagnation)
         self.inner_max_iter = inner_max_iter
         self.preconditioner = preconditioner   # FNO: inner solves become right-preconditioned
+        self.preconditioned_max_iter = 100      # then a plain inner solve takes over
 
     def solve(self, operator, rhs) -> SolverResult:
         cfg = self.cfg
@@ -95,8 +98,15 @@ class MixedBiCGStab:
             convert_axpy(r, 1.0 / rn, r32, False)
             # solve no further than the remaining reduction asks (with a margin)
             want = max(self.inner_rtol, 0.5 * tol / rel)
-            icfg = SolverConfig(max_iter=budget, rtol=min(want, 0.5), chunk=cfg.chunk)
+            icfg = SolverConfig(max_iter=budget if pc is None else min(budget, self.preconditioned_max_iter),
+                                rtol=min(want, 0.5), chunk=cfg.chunk)
             ires = comp.device_bicgstab(r32, icfg, precond=pc)
+            if pc is not None and ires.status != "converged":
+                # the learned preconditioner did not reach the inner tolerance: plain inner solve
+                inner_it += ires.iterations
+                mv32 += ires.matvec_count
+                inner_status.append(f"preconditioned {ires.status}")
+                ires = comp.device_bicgstab(r32, SolverConfig(max_iter=budget, rtol=icfg.rtol, chunk=cfg.chunk))
             inner_it += ires.iterations
             mv32 += ires.matvec_count
             inner_status.append(ires.status)

@@ -176,16 +176,19 @@ int em3d_eta_inclusions(int nx, int ny, int nz, const void* ax, const void* ay, 
 /* ---- learned preconditioner: Fourier neural operator (FNO) ----------------
  * No counterpart in the reference (its solvers are unpreconditioned,
  * solvers/bicgstab.py:12-211).  A network on a grid^3 coarse grid with
- * `channels` hidden channels, `layers` Fourier layers keeping `modes` modes
- * per axis (k in [-modes/2, modes/2)), pointwise channel mixing on bf16 MFMA
- * with fp32 accumulation.  Parameters: fp32 master array of
- * em3d_fno_param_count() floats with a bf16 copy (unsigned short) the
- * forward pass reads; tensors are channels-last [samples][cells][channels]
- * fp32.  work: em3d_fno_work_bytes(), initialised once by
- * em3d_fno_init_work() (twiddle table); a forward call keeps the activations
- * em3d_fno_backward() of the same samples needs. */
+ * `channels` hidden channels and `layers` Fourier layers keeping `modes`
+ * modes per axis (k in [-modes/2, modes/2)), linear in its in_channels
+ * inputs and conditioned on material_channels inputs through multiplicative
+ * gates:  v0 = W_in x,  v_{l+1} = sigmoid(U_l m + d_l) * (W_l v_l + S_l v_l),
+ * y = W_out v_L.  Pointwise channel mixing runs on bf16 MFMA with fp32
+ * accumulation.  Parameters: fp32 master array of em3d_fno_param_count()
+ * floats with a bf16 copy (unsigned short) the forward pass reads; inputs are
+ * channels-last [samples][cells][in_channels + material_channels] fp32,
+ * outputs [samples][cells][out_channels].  work: em3d_fno_work_bytes(),
+ * initialised once by em3d_fno_init_work() (twiddle table); a forward call
+ * keeps the activations em3d_fno_backward() of the same samples needs. */
 typedef struct {
-  int grid, channels, modes, layers, in_channels, out_channels;
+  int grid, channels, modes, layers, in_channels, out_channels, material_channels;
 } em3d_fno_config;
 
 long long em3d_fno_param_count(const em3d_fno_config* config);
@@ -208,9 +211,12 @@ int em3d_fno_to_bf16(long long n, const float* params, unsigned short* params_bf
 
 /* Preconditioner M r = r + s P(y),  y = FNO(R r / s, chi features),
  * s = |r| / sqrt(ncomp N^3) per right-hand side, R = mean over factor^3
- * fine cells, P = copy to them.  Grids are (grid * factor)^3.
- * chi_features: device [cells][chi_channels] from em3d_fno_chi_features
- * (means of the diagonal of chi: 6 channels EM, 2 acoustic);
+ * fine cells, P = trilinear interpolation back.  The network runs on two real samples per
+ * right-hand side (real parts, imaginary parts), so M is complex-linear.
+ * Grids are (grid * factor)^3; in_channels = out_channels = ncomp,
+ * material_channels = chi_channels; work and features are sized for
+ * 2 batch samples.
+ * chi_features: device [cells][chi_channels] from em3d_fno_chi_features;
  * features: em3d_fno_features_bytes(config, batch); work as above for
  * `batch` samples. */
 typedef struct {
@@ -230,13 +236,24 @@ size_t em3d_fno_features_bytes(const em3d_fno_config* config, int samples);
 size_t em3d_fno_scale_work_bytes(int batch);
 int em3d_fno_scale(int batch, long long n, int precision, const void* field, double* scale, void* work,
                    hipStream_t stream);
-/* out[b][cell] = (mean over the cell's factor^3 fine cells of field_b / scale[b]
- * as re, im per component, then the chi_channels chi features) */
+/* out[2b + p][cell] = (mean over the cell's factor^3 fine cells of the real
+ * (p = 0) or imaginary (p = 1) part of field_b / scale[b] per component, then
+ * the chi_channels chi features) */
 int em3d_fno_restrict(int batch, int ncomp, int nx, int ny, int nz, int factor, int precision, const void* field,
                       const double* scale, const float* chi_features, int chi_channels, float* out,
                       hipStream_t stream);
+/* chi features: means of the diagonal of chi over each coarse cell (re, im;
+ * 3 entries EM, 1 acoustic), then, when chi_channels is odd, one constant
+ * channel `extra` (the wave number feature). */
 int em3d_fno_chi_features(int nx, int ny, int nz, int factor, int precision, const void* chi, int chi_kind,
-                          int chi_channels, float* out, hipStream_t stream);
+                          int chi_channels, double extra, float* out, hipStream_t stream);
+/* out = base + scale[b] P(y_2b + i y_2b+1) (base may be NULL): P is the
+ * trilinear prolongation of the preconditioner; y [2 batch][cells][ncomp].
+ * em3d_fno_prolong_adjoint: out[2b + p] = P^T (Re / Im g_b) / scale[b]. */
+int em3d_fno_prolong(int batch, int ncomp, int nx, int ny, int nz, int factor, int precision, const void* base,
+                     const double* scale, const float* y, void* out, hipStream_t stream);
+int em3d_fno_prolong_adjoint(int batch, int ncomp, int nx, int ny, int nz, int factor, int precision, const void* g,
+                             const double* scale, float* out, hipStream_t stream);
 int em3d_fno_precondition(const em3d_fno_model* model, int batch, int ncomp, int nx, int ny, int nz, int precision,
                           const void* r, void* out, hipStream_t stream);
 
