@@ -48,6 +48,10 @@ SIGNATURES = {
     "em3d_rmatvec": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
     "em3d_plan_set_timing": (_I, [_VP, _I]),
     "em3d_plan_timing": (_I, [_VP, C.POINTER(_D), _I, C.POINTER(_I), C.POINTER(_LL)]),
+    "em3d_plan_variant_count": (_I, [_VP, _I]),
+    "em3d_plan_set_variants": (_I, [_VP, C.POINTER(C.c_int)]),
+    "em3d_plan_get_variants": (_I, [_VP, C.POINTER(C.c_int)]),
+    "em3d_plan_autotune": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _I, C.POINTER(C.c_double), _VP]),
     "em3d_bicgstab_work_bytes": (_SZ, [_VP]),
     "em3d_bicgstab": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP, C.POINTER(BiCGStabParams), _VP, _VP, _I,
                            C.POINTER(SolverResultC), _VP]),

@@ -166,6 +166,7 @@ struct ZArgs {
   int batch;
   int perm;                    // doubled axis in mirror-interleaved order (output of a
                                // forward pass, input of an inverse pass)
+  int variant;         // fast-kernel variant chosen by the plan (-1: the generic kernels)
   const int* skip;             // device flag: when non-zero the whole pass is a no-op
 };
 
@@ -241,6 +242,7 @@ struct SArgs {
   AxisPlan plan;
   const void* tw;
   int inv;
+  int variant;         // fast-kernel variant chosen by the plan (-1: the generic kernels)
   const int* skip;             // device flag: when non-zero the whole pass is a no-op
 };
 
@@ -295,6 +297,7 @@ struct XArgs {
   int adj;
   int batch;
   int perm;            // z axis of W2 in mirror-interleaved order
+  int variant;         // fast-kernel variant chosen by the plan (-1: the generic kernels)
   const int* skip;     // device flag: when non-zero the whole pass is a no-op
 };
 

@@ -283,6 +283,76 @@ int em3d_plan_timing(em3d_plan_t plan, double* ms_per_launch, int max_launches, 
   return 0;
 }
 
+int em3d_plan_variant_count(em3d_plan_t plan, int kind) {
+  if (!plan || kind < 0 || kind > 2) return 0;
+  const int axis = kind == PASS_S ? 1 : (kind == PASS_X ? 0 : 2);
+  return fast_variant_count(kind, plan->p.L[axis], plan->p.prec);
+}
+
+int em3d_plan_set_variants(em3d_plan_t plan, const int* variants) {
+  if (!plan || !variants) return fail("em3d_plan_set_variants: null argument");
+  for (int k = 0; k < 3; ++k)
+    if (variants[k] < -1 || variants[k] >= em3d_plan_variant_count(plan, k))
+      return fail("em3d_plan_set_variants: variant " + std::to_string(variants[k]) + " out of range for pass kind " +
+                  std::to_string(k));
+  for (int k = 0; k < 3; ++k) plan->p.variant[k] = variants[k];
+  return 0;
+}
+
+int em3d_plan_get_variants(em3d_plan_t plan, int* variants) {
+  if (!plan || !variants) return fail("em3d_plan_get_variants: null argument");
+  for (int k = 0; k < 3; ++k) variants[k] = plan->p.variant[k];
+  return 0;
+}
+
+int em3d_plan_autotune(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
+                       void* work, int reps, double* best_ms, hipStream_t stream) {
+  if (!plan || !khat || !u || !y || !work) return fail("em3d_plan_autotune: null argument");
+  if (reps < 1) reps = 1;
+  hipEvent_t e0, e1;
+  if (hip_check(hipEventCreate(&e0), "hipEventCreate") || hip_check(hipEventCreate(&e1), "hipEventCreate")) return 1;
+  const bool timing = plan->timing;
+  plan->timing = false;
+  int rc = 0;
+  double best_all = 0.0;
+  auto measure = [&](double* ms) -> int {
+    if (apply_plan(plan, khat, chi, chi_kind, u, y, work, stream, false, "em3d_plan_autotune", nullptr)) return 1;
+    if (hip_check(hipEventRecord(e0, stream), "hipEventRecord")) return 1;
+    for (int r = 0; r < reps; ++r)
+      if (apply_plan(plan, khat, chi, chi_kind, u, y, work, stream, false, "em3d_plan_autotune", nullptr)) return 1;
+    if (hip_check(hipEventRecord(e1, stream), "hipEventRecord") || hip_check(hipEventSynchronize(e1), "sync"))
+      return 1;
+    float t = 0.f;
+    if (hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime")) return 1;
+    *ms = (double)t / reps;
+    return 0;
+  };
+  // coordinate search: the x pass first (it dominates), then the strided and z passes
+  const int order[3] = {PASS_X, PASS_S, PASS_Z};
+  for (int oi = 0; oi < 3 && !rc; ++oi) {
+    const int k = order[oi];
+    const int n = em3d_plan_variant_count(plan, k);
+    int best_v = plan->p.variant[k];
+    double best = -1.0;
+    for (int v = -1; v < n && !rc; ++v) {
+      plan->p.variant[k] = v;
+      double ms = 0.0;
+      rc = measure(&ms);
+      if (!rc && (best < 0.0 || ms < best)) {
+        best = ms;
+        best_v = v;
+      }
+    }
+    plan->p.variant[k] = best_v;
+    best_all = best;
+  }
+  plan->timing = timing;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc && best_ms) *best_ms = best_all;
+  return rc;
+}
+
 int em3d_plan_destroy(em3d_plan_t plan) {
   if (!plan) return 0;
   int rc = 0;

@@ -356,11 +356,18 @@ template <> struct XRadix<192> { static constexpr int A = 4, B = 4, C = 12; };
 template <> struct XRadix<320> { static constexpr int A = 4, B = 4, C = 20; };
 template <> struct XRadix<384> { static constexpr int A = 4, B = 4, C = 24; };
 
-template <typename T, int L, int C, int TC_ = 128 / (int)sizeof(cpx<T>), bool STAGE_ = true, int WAVES_ = 2>
+// an explicit radix plan (A, B, C) of the x pass
+template <int A_, int B_, int C_>
+struct XR {
+  static constexpr int A = A_, B = B_, C = C_;
+};
+
+template <typename T, int L, int C, int TC_ = 128 / (int)sizeof(cpx<T>), bool STAGE_ = true, int WAVES_ = 2,
+          class RX = XRadix<L>>
 struct FastX {
   using Real = T;
   static constexpr int NCOMP = C;
-  static constexpr int RA = XRadix<L>::A, RB = XRadix<L>::B, RC = XRadix<L>::C;
+  static constexpr int RA = RX::A, RB = RX::B, RC = RX::C;
   static constexpr int NTC = L / RC;                   // threads per column
   static constexpr int S = 64 / (int)sizeof(cpx<T>);   // octant group: 64 bytes of iz
   static constexpr int TC = TC_, TCP = TC + 1;        // columns per block, TC divides 2S
@@ -783,9 +790,29 @@ struct FastZI {
 // zf<K>(const ZArgs&), zi<K>(const ZArgs&).  Returns false when no fast
 // kernel covers the pass (the caller then uses the generic kernels).
 // ---------------------------------------------------------------------------
-template <typename T, int L, class Run>
-bool fast_s_len(Run& run, const SArgs& a) {
-  using Cfg = FastCfg<T, L>;
+// Kernel variants the plan may choose per pass kind (the autotuner times
+// them): 0 is the default configuration; alternatives exist for the
+// single-precision lengths that dominate the measured workloads.
+inline int fast_variant_count(int kind, int L, int prec) {
+  if (prec != 0) return 1;
+  if (kind == 1) return (L == 256) ? 4 : (L == 128 || L == 192) ? 3 : 1;   // x pass
+  if (kind == 0) return (L == 256 || L == 192) ? 3 : 1;                      // strided passes
+  return (L == 256 || L == 192) ? 2 : 1;                                      // z passes
+}
+
+// strided-pass configuration of a variant: (R1, TC)
+template <typename T, int L, int V>
+struct SVar {
+  static constexpr int R1 = FastCfg<T, L>::R1, TC = FastCfg<T, L>::TC;
+};
+template <> struct SVar<float, 256, 1> { static constexpr int R1 = 32, TC = 16; };
+template <> struct SVar<float, 256, 2> { static constexpr int R1 = 16, TC = 32; };
+template <> struct SVar<float, 192, 1> { static constexpr int R1 = 48, TC = 32; };
+template <> struct SVar<float, 192, 2> { static constexpr int R1 = 24, TC = 16; };
+
+template <typename T, int L, int V, class Run>
+bool fast_s_var(Run& run, const SArgs& a) {
+  using Cfg = SVar<T, L, V>;
   if (a.n_in * 2 == L && a.n_out == L && !a.inv) {
     run.template s<FastS<T, L, Cfg::R1, Cfg::TC, false, true, false>>(a);
     return true;
@@ -803,6 +830,15 @@ bool fast_s_len(Run& run, const SArgs& a) {
   return false;
 }
 
+template <typename T, int L, class Run>
+bool fast_s_len(Run& run, const SArgs& a) {
+  if constexpr (std::is_same<T, float>::value && (L == 256 || L == 192)) {
+    if (a.variant == 1) return fast_s_var<T, L, 1>(run, a);
+    if (a.variant == 2) return fast_s_var<T, L, 2>(run, a);
+  }
+  return fast_s_var<T, L, 0>(run, a);
+}
+
 // x-pass tile of the 3-component operator: columns per block, octant run
 // staged in LDS, minimum waves per SIMD (register budget)
 #ifndef E3_X3_TC
@@ -815,26 +851,53 @@ bool fast_s_len(Run& run, const SArgs& a) {
 #define E3_X3_WAVES 2
 #endif
 
-template <typename T, int L, class Run>
-bool fast_x_len(Run& run, const XArgs& a) {
+template <typename T, int L, class RX, int TC, class Run>
+bool fast_x3(Run& run, const XArgs& a) {
   // mixed-radix lengths hold more live values: one wave per SIMD keeps them
   // in registers (VGPR + AGPR) instead of scratch
-  using X3 = FastX<T, L, 3, E3_X3_TC, E3_X3_STAGE, is_pow2(L) ? E3_X3_WAVES : 1>;
-  using X1 = FastX<T, L, 1>;
-  if (a.N[0] * 2 != L || a.L[2] % X3::TC != 0 || !a.perm) return false;
-  if (a.C == 3) {
-    if (X3::NT > 1024) return false;
-    run.template x<X3>(a);
-  } else {
-    if (X1::NT > 1024) return false;
-    run.template x<X1>(a);
-  }
+  using X3 = FastX<T, L, 3, TC, E3_X3_STAGE, is_pow2(L) ? E3_X3_WAVES : 1, RX>;
+  if (a.L[2] % X3::TC != 0 || X3::NT > 1024) return false;
+  run.template x<X3>(a);
   return true;
 }
 
 template <typename T, int L, class Run>
-bool fast_z_len(Run& run, const ZArgs& a) {
-  using Cfg = FastCfg<T, L>;
+bool fast_x_len(Run& run, const XArgs& a) {
+  using X1 = FastX<T, L, 1>;
+  if (a.N[0] * 2 != L || !a.perm) return false;
+  if (a.C == 3) {
+    constexpr int TCd = E3_X3_TC;
+    if constexpr (std::is_same<T, float>::value && L == 256) {
+      if (a.variant == 1) return fast_x3<T, L, XR<4, 4, 16>, TCd>(run, a);
+      if (a.variant == 2) return fast_x3<T, L, XR<8, 4, 8>, 8>(run, a);
+      if (a.variant == 3) return fast_x3<T, L, XR<16, 1, 16>, TCd>(run, a);
+    }
+    if constexpr (std::is_same<T, float>::value && L == 128) {
+      if (a.variant == 1) return fast_x3<T, L, XR<8, 2, 8>, TCd>(run, a);
+      if (a.variant == 2) return fast_x3<T, L, XR<8, 1, 16>, TCd>(run, a);
+    }
+    if constexpr (std::is_same<T, float>::value && L == 192) {
+      if (a.variant == 1) return fast_x3<T, L, XR<4, 2, 24>, TCd>(run, a);
+      if (a.variant == 2) return fast_x3<T, L, XR<8, 1, 24>, TCd>(run, a);
+    }
+    return fast_x3<T, L, XRadix<L>, TCd>(run, a);
+  }
+  if (a.L[2] % X1::TC != 0 || X1::NT > 1024) return false;
+  run.template x<X1>(a);
+  return true;
+}
+
+// z-pass configuration of a variant: (R1, P)
+template <typename T, int L, int V>
+struct ZVar {
+  static constexpr int R1 = FastCfg<T, L>::R1, P = FastCfg<T, L>::P;
+};
+template <> struct ZVar<float, 256, 1> { static constexpr int R1 = 32, P = 32; };
+template <> struct ZVar<float, 192, 1> { static constexpr int R1 = 48, P = 32; };
+
+template <typename T, int L, int V, class Run>
+bool fast_z_var(Run& run, const ZArgs& a) {
+  using Cfg = ZVar<T, L, V>;
   if (a.C != 1 && a.C != 3) return false;
   if (a.n_in * 2 == L && a.n_out == L && a.inv == 0 && a.epi == EPI_STORE) {
     run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, true>>(a);
@@ -853,6 +916,14 @@ bool fast_z_len(Run& run, const ZArgs& a) {
     }
   }
   return false;
+}
+
+template <typename T, int L, class Run>
+bool fast_z_len(Run& run, const ZArgs& a) {
+  if constexpr (std::is_same<T, float>::value && (L == 256 || L == 192)) {
+    if (a.variant == 1) return fast_z_var<T, L, 1>(run, a);
+  }
+  return fast_z_var<T, L, 0>(run, a);
 }
 
 #define E3_FAST_SWITCH(FN, T, RUN, ARGS)        \
@@ -886,11 +957,20 @@ bool fast_z_len(Run& run, const ZArgs& a) {
   }
 
 template <typename T, class Run>
-bool fast_s(Run& run, const SArgs& a) { E3_FAST_SWITCH_MIXED(fast_s_len, T, run, a) }
+bool fast_s(Run& run, const SArgs& a) {
+  if (a.variant < 0) return false;
+  E3_FAST_SWITCH_MIXED(fast_s_len, T, run, a)
+}
 template <typename T, class Run>
-bool fast_x(Run& run, const XArgs& a) { E3_FAST_SWITCH_MIXED(fast_x_len, T, run, a) }
+bool fast_x(Run& run, const XArgs& a) {
+  if (a.variant < 0) return false;
+  E3_FAST_SWITCH_MIXED(fast_x_len, T, run, a)
+}
 template <typename T, class Run>
-bool fast_z(Run& run, const ZArgs& a) { E3_FAST_SWITCH_MIXED(fast_z_len, T, run, a) }
+bool fast_z(Run& run, const ZArgs& a) {
+  if (a.variant < 0) return false;
+  E3_FAST_SWITCH_MIXED(fast_z_len, T, run, a)
+}
 #undef E3_FAST_SWITCH
 #undef E3_FAST_SWITCH_MIXED
 

@@ -30,7 +30,10 @@ struct Plan {
   long long khat_elems;           // octant spectrum storage elements (with group padding)
   long long khat_entries;         // logical octant entries K (Nx+1)(Ny+1)(Nz+1)
   long long build_elems;          // complex<double> scratch of the kernel build
+  int variant[3];                 // kernel variant per pass kind (strided, x, z); -1: generic kernels
 };
+
+enum PassKind { PASS_S = 0, PASS_X = 1, PASS_Z = 2 };
 
 inline size_t elem_bytes(int prec) { return prec == PREC_DOUBLE ? 16 : 8; }
 
@@ -123,7 +126,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.out_ps = Lz; z.out_cs = P * Lz; z.out_bs = C * P * Lz;
   z.n_in = Nz; z.n_out = Lz; z.inv = 0;
   z.pro = adjoint ? PRO_COPY : PRO_CHI; z.epi = EPI_STORE;
-  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.perm = 1; z.skip = skip;
+  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.perm = 1; z.skip = skip; z.variant = p.variant[PASS_Z];
   z_pass(go, p, prec, twz, p.ax[2], z);
 
   // P2: y forward (Ny live rows -> Ly)
@@ -131,7 +134,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   std::memset(&s, 0, sizeof(s));
   s.in = W1; s.out = W2; s.Q = (long long)p.batch * C * Nx; s.W = Lz;
   s.in_qs = (long long)Ny * Lz; s.out_qs = (long long)Ly * Lz;
-  s.n_in = Ny; s.n_out = Ly; s.inv = 0; s.skip = skip;
+  s.n_in = Ny; s.n_out = Ly; s.inv = 0; s.skip = skip; s.variant = p.variant[PASS_S];
   s_pass(go, prec, twy, p.ax[1], s);
 
   // P3: x forward, spectral multiply, x inverse (Nx rows kept), in place
@@ -140,6 +143,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   x.data = W2; x.C = p.C; x.W = (long long)Ly * Lz; x.cs = (long long)Nx * x.W; x.bs = C * x.cs;
   for (int a = 0; a < 3; ++a) { x.N[a] = p.N[a]; x.L[a] = p.L[a]; }
   x.plan = p.ax[0]; x.tw = twx; x.khat = khat; x.adj = adjoint ? 1 : 0; x.batch = p.batch; x.perm = 1; x.skip = skip;
+  x.variant = p.variant[PASS_X];
   x.ps = Lx + 1;
   x.T = tile_count(lds_bytes(p.C, x.ps, prec), 16);
   if (x.T > x.W) x.T = (int)x.W;
@@ -157,7 +161,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.out_ps = Nz; z.out_cs = P * Nz; z.out_bs = C * P * Nz;
   z.n_in = Lz; z.n_out = Nz; z.inv = 1;
   z.pro = PRO_COPY; z.epi = adjoint ? EPI_SUB_CHIH : EPI_SUB;
-  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.u = u; z.perm = 1; z.skip = skip;
+  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.u = u; z.perm = 1; z.skip = skip; z.variant = p.variant[PASS_Z];
   z_pass(go, p, prec, twz, p.ax[2], z);
 }
 

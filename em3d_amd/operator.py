@@ -60,6 +60,18 @@ class HipPlan:
         _lib.check(lib.em3d_plan_sizes(h, C.byref(kb), C.byref(wb), C.byref(bb)), "em3d_plan_sizes")
         self.khat_bytes, self.work_bytes, self.build_bytes = kb.value, wb.value, bb.value
 
+    def variants(self) -> list[int]:
+        v = (C.c_int * 3)()
+        _lib.check(_lib.lib().em3d_plan_get_variants(self.handle, v), "em3d_plan_get_variants")
+        return [int(x) for x in v]
+
+    def set_variants(self, variants) -> None:
+        v = (C.c_int * 3)(*[int(x) for x in variants])
+        _lib.check(_lib.lib().em3d_plan_set_variants(self.handle, v), "em3d_plan_set_variants")
+
+    def variant_counts(self) -> list[int]:
+        return [int(_lib.lib().em3d_plan_variant_count(self.handle, k)) for k in range(3)]
+
     @classmethod
     def get(cls, N, ncomp, batch, precision, device_index=0) -> "HipPlan":
         key = (tuple(int(n) for n in N), int(ncomp), int(batch), precision, int(device_index))
@@ -355,6 +367,27 @@ class _FieldApplier:
                                     residual_action_counts=counts, status=status,
                                     true_final_residual=float(history[-1]) if history else float("inf")))
         return out if batched else out[0]
+
+    def autotune(self, reps: int = 5, batch: int = 1):
+        """Pick the fastest FFT kernel variant of each pass kind for this grid
+        shape, precision and batch by timing whole matvecs (libem3d_hip's
+        em3d_plan_autotune); the choice is kept by the shared plan.  Returns
+        {"variants": [strided, x, z], "ms": time per matvec}."""
+        import torch
+        be = self._be
+        plan = HipPlan.get(self._N, self._ncomp, batch, be.precision, be.device_index)
+        khat, kind, chi = self._spectrum()
+        shape = ((batch,) if batch > 1 else ()) + self._field_shape()
+        g = torch.Generator(device=be.torch_device).manual_seed(0)
+        u = torch.randn(shape, dtype=be.precision.torch_complex_dtype, device=be.torch_device, generator=g)
+        y = torch.empty_like(u)
+        best = C.c_double()
+        lib = _lib.lib()
+        _lib.check(lib.em3d_plan_autotune(plan.handle, khat.data_ptr(), chi.data_ptr() if chi is not None else None,
+                                          kind, u.data_ptr(), y.data_ptr(), self._workspace(plan).data_ptr(),
+                                          int(reps), C.byref(best), _lib.stream_handle(be.torch_device)),
+                   "em3d_plan_autotune")
+        return {"variants": plan.variants(), "ms": best.value}
 
     def _apply(self, field, khat, chi_kind, chi, adjoint: bool):
         import torch

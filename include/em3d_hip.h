@@ -88,6 +88,21 @@ int em3d_rmatvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_ki
 int em3d_plan_set_timing(em3d_plan_t plan, int enable);
 int em3d_plan_timing(em3d_plan_t plan, double* ms_per_launch, int max_launches, int* nlaunch, long long* calls);
 
+/* FFT autotuner (no counterpart in the reference, whose FFTs are
+ * numpy/cuFFT calls, operator.py:353-363).  Each pass kind (0 strided y
+ * passes, 1 x pass, 2 z passes) runs one of em3d_plan_variant_count() fast
+ * kernel variants (radix plans, tile widths, pencils per block; variant 0 is
+ * the default) or, with -1, the generic LDS kernels.  em3d_plan_autotune()
+ * times whole matvecs on the caller's buffers (`reps` after one warm-up, per
+ * candidate, coordinate search x -> strided -> z), keeps the fastest variant
+ * of each kind in the plan and returns its time per matvec.  Synchronous;
+ * y and work are overwritten. */
+int em3d_plan_variant_count(em3d_plan_t plan, int kind);
+int em3d_plan_set_variants(em3d_plan_t plan, const int* variants);
+int em3d_plan_get_variants(em3d_plan_t plan, int* variants);
+int em3d_plan_autotune(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
+                       void* work, int reps, double* best_ms, hipStream_t stream);
+
 /* Device-resident BiCGStab on the operator of a batch-1 plan
  * (BiCGStab.solve, solvers/bicgstab.py:12-211, with SolverConfig fields of
  * solvers/base.py:7-25).  Same recurrence, early exit on ||s||, statuses and

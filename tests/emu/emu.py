@@ -39,6 +39,8 @@ def load(path: str | None = None):
     lib.emu_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                               C.c_void_p, C.c_int]
     lib.emu_last_error.restype = C.c_char_p
+    lib.emu_plan_variant_count.argtypes = [C.c_void_p, C.c_int]
+    lib.emu_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.emu_set_fast.argtypes = [C.c_int]
     lib.emu_fast_calls.argtypes = [C.c_int]
     lib.emu_fast_calls.restype = C.c_longlong

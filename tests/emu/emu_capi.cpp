@@ -227,6 +227,13 @@ void* emu_plan_create(int nx, int ny, int nz, int ncomp, int batch, int precisio
 
 void emu_plan_destroy(void* h) { delete (EmuPlan*)h; }
 
+int emu_plan_variant_count(void* h, int kind) {
+  const Plan& p = ((EmuPlan*)h)->p;
+  return fast_variant_count(kind, p.L[kind == PASS_S ? 1 : (kind == PASS_X ? 0 : 2)], p.prec);
+}
+
+void emu_plan_set_variant(void* h, int kind, int v) { ((EmuPlan*)h)->p.variant[kind] = v; }
+
 void emu_plan_sizes(void* h, long long* khat_elems, long long* work_elems, long long* build_elems) {
   const Plan& p = ((EmuPlan*)h)->p;
   *khat_elems = p.khat_elems;

@@ -357,3 +357,27 @@ def test_device_materials_match_host(em, prec):
         m = eta_ellipsoid(grid, center=c0, radii=r, eta_inside=1.0, eta_outside=0.0) != 0
         want = torch.where(m, torch.tensor(v, dtype=eta.dtype, device=eta.device), want)
     assert torch.equal(eta, want)
+
+
+def test_autotuner_keeps_results(em):
+    """The autotuner picks valid variants and the matvec it leaves behind
+    agrees with the default kernels (single precision, 1e-6)."""
+    import torch
+    from em3d_amd.operator import HipPlan
+    N = (64, 64, 64)
+    _say("[autotune] 64^3 single")
+    be = _be(em, "single")
+    prob = _config2(em, be, N, 8.0)
+    op = em.Operator(prob)
+    plan = HipPlan.get(N, 3, 1, em.Precision.SINGLE)
+    plan.set_variants([0, 0, 0])
+    ref = op.matvec(prob.wave)
+    counts = plan.variant_counts()
+    out = op.autotune(reps=2)
+    v = out["variants"]
+    assert all(-1 <= v[k] < counts[k] for k in range(3)) and out["ms"] > 0
+    got = op.matvec(prob.wave)
+    assert float(torch.linalg.vector_norm(got - ref) / torch.linalg.vector_norm(ref)) < 1e-6
+    with pytest.raises(Exception, match="out of range"):
+        plan.set_variants([counts[0], 0, 0])
+    plan.set_variants([0, 0, 0])
