@@ -353,6 +353,112 @@ int em3d_plan_autotune(em3d_plan_t plan, const void* khat, const void* chi, int 
   return rc;
 }
 
+// ---- packed groups of operators replayed from one captured hipGraph ----------
+}  // extern "C"
+
+struct em3d_group_s {
+  struct Member {
+    em3d_plan_t plan;
+    const void* khat;
+    const void* chi;
+    int chi_kind;
+    const void* u;
+    void* y;
+    void* work;
+  };
+  std::vector<Member> m;
+  int device = 0;
+  hipStream_t cap = nullptr;       // private capture stream (default priority)
+  std::vector<hipStream_t> branch; // one stream per member: parallel branches of the graph
+  std::vector<hipEvent_t> ev;      // fork event + one join event per member
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int em3d_group_destroy(em3d_group_t g);
+
+int em3d_group_create(int n, const em3d_plan_t* plans, const void* const* khats, const void* const* chis,
+                      const int* chi_kinds, const void* const* inputs, void* const* outputs, void* const* works,
+                      em3d_group_t* group) {
+  if (n < 1 || !plans || !khats || !chis || !chi_kinds || !inputs || !outputs || !works || !group)
+    return fail("em3d_group_create: null argument or empty group");
+  auto* g = new em3d_group_s();
+  for (int i = 0; i < n; ++i) {
+    if (!plans[i] || !khats[i] || !inputs[i] || !outputs[i] || !works[i]) {
+      delete g;
+      return fail("em3d_group_create: null member " + std::to_string(i));
+    }
+    g->m.push_back({plans[i], khats[i], chis[i], chi_kinds[i], inputs[i], outputs[i], works[i]});
+  }
+  int rc = hip_check(hipGetDevice(&g->device), "hipGetDevice");
+  if (!rc) rc = hip_check(hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  g->branch.assign(n, nullptr);
+  g->ev.assign(n + 1, nullptr);
+  for (int i = 0; i < n && !rc; ++i)
+    rc = hip_check(hipStreamCreateWithFlags(&g->branch[i], hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  for (int i = 0; i <= n && !rc; ++i)
+    rc = hip_check(hipEventCreateWithFlags(&g->ev[i], hipEventDisableTiming), "hipEventCreateWithFlags");
+  if (rc) {
+    em3d_group_destroy(g);
+    return 1;
+  }
+  *group = g;
+  return 0;
+}
+
+// y_i = A_i u_i for every member.  The first call captures the members'
+// pass sequences (no memsets, no events) into one graph; every call then
+// launches that graph on `stream`.  Inputs and outputs are the buffers given
+// at creation.
+int em3d_group_matvec(em3d_group_t g, hipStream_t stream) {
+  if (!g) return fail("em3d_group_matvec: null group");
+  if (!g->exec) {
+    std::vector<bool> timing(g->m.size());
+    for (size_t i = 0; i < g->m.size(); ++i) {
+      timing[i] = g->m[i].plan->timing;
+      g->m[i].plan->timing = false;
+    }
+    // fork: every member on its own branch stream, joined back into the capture stream
+    int rc = hip_check(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (!rc) rc = hip_check(hipEventRecord(g->ev[0], g->cap), "hipEventRecord(fork)");
+    for (size_t i = 0; i < g->m.size() && !rc; ++i) {
+      const auto& q = g->m[i];
+      rc = hip_check(hipStreamWaitEvent(g->branch[i], g->ev[0], 0), "hipStreamWaitEvent(fork)");
+      if (!rc)
+        rc = apply_plan(q.plan, q.khat, q.chi, q.chi_kind, q.u, q.y, q.work, g->branch[i], false,
+                        "em3d_group_matvec", nullptr);
+      if (!rc) rc = hip_check(hipEventRecord(g->ev[i + 1], g->branch[i]), "hipEventRecord(join)");
+      if (!rc) rc = hip_check(hipStreamWaitEvent(g->cap, g->ev[i + 1], 0), "hipStreamWaitEvent(join)");
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(g->cap, &graph);
+    for (size_t i = 0; i < g->m.size(); ++i) g->m[i].plan->timing = timing[i];
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return 1;
+    }
+    if (hip_check(e, "hipStreamEndCapture")) return 1;
+    g->graph = graph;
+    if (hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate")) return 1;
+  }
+  return hip_check(hipGraphLaunch(g->exec, stream), "hipGraphLaunch");
+}
+
+int em3d_group_destroy(em3d_group_t g) {
+  if (!g) return 0;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  for (hipEvent_t e : g->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t b : g->branch)
+    if (b) (void)hipStreamDestroy(b);
+  if (g->cap) (void)hipStreamDestroy(g->cap);
+  delete g;
+  return 0;
+}
+
 int em3d_plan_destroy(em3d_plan_t plan) {
   if (!plan) return 0;
   int rc = 0;

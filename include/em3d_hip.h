@@ -88,6 +88,22 @@ int em3d_rmatvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_ki
 int em3d_plan_set_timing(em3d_plan_t plan, int enable);
 int em3d_plan_timing(em3d_plan_t plan, double* ms_per_launch, int max_launches, int* nlaunch, long long* calls);
 
+/* Packed group of operators of different grid shapes (each with its own
+ * plan, spectrum, contrast and buffers) applied by one hipGraph launch: the
+ * first em3d_group_matvec captures every member's five passes as a parallel
+ * branch of one graph (a private stream per member, forked from and joined
+ * into a capture stream), later calls replay the graph on the caller's
+ * stream, so the grids run concurrently.  y_i = A_i u_i
+ * with the u_i, y_i, work_i given at creation (the graph holds these
+ * pointers).  No counterpart in the reference (one operator per call,
+ * operator.py:353-363). */
+typedef struct em3d_group_s* em3d_group_t;
+int em3d_group_create(int n, const em3d_plan_t* plans, const void* const* khats, const void* const* chis,
+                      const int* chi_kinds, const void* const* inputs, void* const* outputs, void* const* works,
+                      em3d_group_t* group);
+int em3d_group_matvec(em3d_group_t group, hipStream_t stream);
+int em3d_group_destroy(em3d_group_t group);
+
 /* FFT autotuner (no counterpart in the reference, whose FFTs are
  * numpy/cuFFT calls, operator.py:353-363).  Each pass kind (0 strided y
  * passes, 1 x pass, 2 z passes) runs one of em3d_plan_variant_count() fast

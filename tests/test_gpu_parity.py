@@ -381,3 +381,30 @@ def test_autotuner_keeps_results(em):
     with pytest.raises(Exception, match="out of range"):
         plan.set_variants([counts[0], 0, 0])
     plan.set_variants([0, 0, 0])
+
+
+def test_packed_group_graph_replays_match_eager(em):
+    """Grids 16^3 / 24^3 / 32^3 / 40^3 in one captured graph: every replay,
+    with eager work and synchronisations in between and new inputs each time,
+    equals the eager matvecs."""
+    import torch
+    from em3d_amd.packed import PackedOperators
+    _say("[packed] 16/24/32/40 graph replays")
+    be = _be(em, "single")
+    ops = []
+    for i, n in enumerate((16, 24, 32, 40)):
+        prob = _config2(em, be, (n, n, n), 4.0 + i)
+        ops.append(em.Operator(prob))
+    group = PackedOperators(ops, autotune=False)
+    rng = np.random.default_rng(11)
+    for rep in range(3):
+        fields = [be.array(_rand((3,) + op._N, 100 + 10 * rep + j).astype(np.complex64)) for j, op in enumerate(ops)]
+        got = group.matvec(fields)
+        torch.cuda.synchronize()
+        noise = ops[rep].matvec(be.array(_rand((3,) + ops[rep]._N, 7).astype(np.complex64)))   # eager work between
+        torch.cuda.synchronize()
+        for j, op in enumerate(ops):
+            assert torch.equal(got[j], op.matvec(fields[j])), (rep, j)
+        del noise
+    with pytest.raises(ValueError):
+        group.matvec(fields[:2])
