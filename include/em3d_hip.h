@@ -176,6 +176,17 @@ int em3d_axpbypcz(int precision, long long n, double a_re, double a_im, const vo
 int em3d_convert_axpy(long long n, int src_precision, const void* src, double a_re, double a_im, int dst_precision,
                       void* dst, int accumulate, hipStream_t stream);
 
+/* Far-field scatter integral F_b(d_m) = dv sum_q (chi u_b)(r_q)
+ * exp(-i k0 d_m . r_q) (scatter_integral_direct, farfield/_core.py:8-71;
+ * acoustics/farfield.py for ncomp = 1), accumulated in FP64.  u: (batch,
+ * ncomp, N) fields, chi packed as for em3d_matvec (acoustic: chi = eta - 1,
+ * kind 1); dirs: DEVICE (ndir, 3) doubles; out: DEVICE (batch, ndir, ncomp)
+ * complex128; work: em3d_scatter_work_bytes(batch, ndir, ncomp). */
+size_t em3d_scatter_work_bytes(int batch, int ndir, int ncomp);
+int em3d_scatter_integral(int nx, int ny, int nz, const void* ax, const void* ay, const void* az, int precision,
+                          int batch, int ncomp, const void* chi, int chi_kind, const void* u, int ndir,
+                          const double* dirs, double k0, double dv, double* out, void* work, hipStream_t stream);
+
 /* Batch of plane waves f_b = A_b exp(i sign (k o_b . r + phi0_b)), out
  * (batch, 3, nx, ny, nz) in `precision`; ax/ay/az are the grid's cell-centre
  * axes on the device in the precision's real type; params is a DEVICE array of
