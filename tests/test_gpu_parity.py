@@ -439,3 +439,17 @@ def test_gamma0_coarse_operator_and_sim(em):
     res = em.SIM(em.SolverConfig(max_iter=300, rtol=1e-6, mu=a.mu, radius=a.radius)).solve(em.Operator(prob), wave)
     _say(f"[gamma0] mu={a.mu:.4f} R={a.radius:.4f} SIM {res.status} {res.iterations} it")
     assert res.status in ("converged", "max_iter", "divergence_guard") and res.residual_history[-1] < 1.0
+
+
+def test_reference_format_records(em):
+    from em3d_amd import reports
+    be = _be(em, "double")
+    prob = _config2(em, be, (8, 8, 8), 4.0)
+    op = em.Operator(prob)
+    rec = reports.matvec_record("ellipsoid_N8", op, prob.wave, repeats=3, operator_build_sec=0.1)
+    assert list(rec) == ["case_name", "N", "dof", "operator_build_sec", "matvec_avg_sec", "matvec_min_sec",
+                         "matvec_max_sec", "repeats"]
+    assert rec["N"] == "8x8x8" and rec["dof"] == 1536 and rec["matvec_min_sec"] <= rec["matvec_avg_sec"]
+    res, run = reports.timed_solve(em.BiCGStab(em.SolverConfig(rtol=1e-8)), op, prob.wave, case_name="ellipsoid_N8",
+                                   solver_name="BiCGStab")
+    assert run.iterations == res.iterations and run.final_residual == res.residual_history[-1]
