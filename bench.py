@@ -231,6 +231,40 @@ def packed_section(em, sizes=(32, 48, 64, 80), k0=8.0, reps=20):
     return out
 
 
+def acoustic_section(em, sizes=(128, 256), k0=10.0, reps=20):
+    """Acoustic workload: eta = 1 plus 32 seeded spherical inclusions
+    (default_rng(3): radius U[0.03, 0.08], eta U[1.5, 3] + i U[0, 0.2]),
+    k0 = 10, complex64: matvec time and HBM rate, BiCGStab to rtol 1e-6 and
+    the far-field pattern over 360 angles in the xy plane."""
+    import torch
+    from em3d_amd.acoustics import farfield as af
+    from em3d_amd.acoustics.materials import eta_inclusions, random_spheres
+    be = em.Backend.hip(em.Precision.SINGLE)
+    out = {}
+    for N in sizes:
+        grid = em.Grid(N=(N, N, N), L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+        eta = eta_inclusions(grid, random_spheres(32, seed=3))
+        prob = em.acoustics.make_acoustic_problem(grid, eta, k0=k0)
+        op = em.acoustics.AcousticOperator(prob)
+        f = prob.wave
+        op.matvec(f)
+        _, t_mv = _timed(lambda: [op.matvec(f) for _ in range(reps)])
+        ms = t_mv / reps * 1e3
+        M = N ** 3
+        nbytes = (27 + 1 + ((N + 1) / N) ** 3) * M * 8
+        res, t_solve = _timed(lambda: em.BiCGStab(em.SolverConfig(max_iter=500, rtol=1e-6)).solve(op, f))
+        (phi, pat), t_ff = _timed(lambda: af.pattern_plane(res.u, prob, plane="xy", n_angles=360))
+        out[f"N={N}"] = {"matvec_ms": round(ms, 4), "matvec_hbm_gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                         "algorithmic_bytes": int(nbytes), "bicgstab_seconds": round(t_solve, 4),
+                         "iterations": res.iterations, "status": res.status,
+                         "farfield_360_ms": round(t_ff * 1e3, 3)}
+        log(f"[bench] acoustic {N}^3: matvec {ms:.3f} ms, BiCGStab {t_solve:.3f} s ({res.iterations} it), "
+            f"pattern {t_ff * 1e3:.2f} ms")
+        del op, prob, res
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(N, reps, k0):
     """Oracle (NumPy, complex64 as the reference's SINGLE path) matvec at the
     same size; the spectrum is a random tensor of the reference's dense
@@ -267,6 +301,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-acoustic", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="time the default kernel variants")
     ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned preconditioner")
     ap.add_argument("--batch-n", type=int, default=96)
@@ -368,6 +403,10 @@ def main():
         batch = batch_section(em, args.batch_n, args.batch_dirs, 8.0)
         batch["packed"] = packed_section(em)
 
+    acoustic = None
+    if not args.no_acoustic:
+        acoustic = acoustic_section(em)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, numpy)")
@@ -405,6 +444,7 @@ def main():
                      "algorithmic_bytes": pb[dom]},
         "solve": solve,
         "batch": batch,
+        "acoustic": acoustic,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
