@@ -291,6 +291,30 @@ def cpu_baseline(N, reps, k0):
                       f"os.cpu_count()={os.cpu_count()}"}
 
 
+def profile_case(args):
+    """Matvec loop of one workload (warm-up + steps), printing its time: the
+    FP64 EM matvec at N^3 (config-2 ellipsoid) or the acoustic 256^3 matvec
+    (32 inclusions, k0 = 10), complex64."""
+    import torch
+
+    import em3d_amd as em
+    if args.case == "em-fp64":
+        be = em.Backend.hip(em.Precision.DOUBLE)
+        prob, _ = make_problem(em, be, args.n, args.k0)
+        op = em.Operator(prob)
+    else:
+        from em3d_amd.acoustics.materials import eta_inclusions, random_spheres
+        be = em.Backend.hip(em.Precision.SINGLE)
+        grid = em.Grid(N=(256,) * 3, L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+        prob = em.acoustics.make_acoustic_problem(grid, eta_inclusions(grid, random_spheres(32, seed=3)), k0=10.0)
+        op = em.acoustics.AcousticOperator(prob)
+    f = prob.wave
+    for _ in range(args.warmup):
+        op.matvec(f)
+    _, t = _timed(lambda: [op.matvec(f) for _ in range(args.steps)])
+    print(json.dumps({"case": args.case, "ms_per_matvec": round(t / args.steps * 1e3, 4)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +324,8 @@ def main():
     ap.add_argument("--k0", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--case", default="full", choices=["full", "em-fp64", "acoustic-256"],
+                    help="full: the benchmark; em-fp64 / acoustic-256: only that matvec, for profiling")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-acoustic", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="time the default kernel variants")
@@ -312,6 +338,8 @@ def main():
                     help="also run 3 plain device copies of 512 MiB (read 512 MiB, write 512 MiB each), "
                          "the calibration kernel for HBM traffic counters")
     args = ap.parse_args()
+    if args.case != "full":
+        return profile_case(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: the path runs on one "
               "GPU (right-hand sides and materials would shard one process per GPU)", file=sys.stderr)
