@@ -26,6 +26,17 @@
 #define E3_SCHED_FENCE() ((void)0)
 #endif
 
+// E3_OPAQUE(x): an empty volatile asm that takes and returns the integer x
+// in a vector register.  Values computed from x after this point cannot be
+// hoisted above it, in particular not out of a loop, so per-lane addresses
+// are formed in the phase that uses them instead of being held in registers
+// across the whole kernel.  No instruction is emitted; host build: nothing.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define E3_OPAQUE(x) asm volatile("" : "+v"(x))
+#else
+#define E3_OPAQUE(x) ((void)0)
+#endif
+
 namespace e3 {
 
 template <typename T>

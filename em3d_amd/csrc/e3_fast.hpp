@@ -441,7 +441,8 @@ struct FastX {
   // ---- forward stage A (Ns = 1): global rows < L/2 -> regs -> LDS ----
   // stage-A butterfly jj = q.j + bb NTC reads rows jj + m L/RA, live for m < RA/2
   E3_HD void fwd_a(int c, int tid, int h, St& s) const {
-    const Lane q = lane(tid, h);
+    Lane q = lane(tid, h);
+    E3_OPAQUE(q.j);   // row addresses are formed per phase, not held across the row
     cpx<T>* t = seg(q.h);
     const cpx<T>* d = plane(c);
 #pragma unroll
@@ -560,7 +561,9 @@ struct FastX {
                : *at_offset(run, (unsigned)(((ix * K + comp) * S + slot) * sizeof(cpx<T>)));
   }
   E3_HD void multiply(int tid, int h, St& s) const {
-    const Lane q = lane(tid, h);
+    Lane q = lane(tid, h);
+    E3_OPAQUE(q.j);   // the 6 RC octant addresses are formed here, not held across the row
+    E3_OPAQUE(q.col);
     const int pos = pos0 + q.col;
     const bool nyq = pos == 1;                          // kz = Nz
     const int slot = (pos >> 1) % S;
@@ -607,7 +610,8 @@ struct FastX {
   }
   // ---- inverse last stage (radix RA, Ns = L/RA): LDS -> rows < Nx ----
   E3_HD void inv_a(int c, int tid, int h) const {
-    const Lane q = lane(tid, h);
+    Lane q = lane(tid, h);
+    E3_OPAQUE(q.j);
     const cpx<T>* t = seg(q.h);
     const cpx<T>* tw = twl();
     cpx<T>* d = plane(c);
