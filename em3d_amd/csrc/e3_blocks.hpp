@@ -304,19 +304,19 @@ struct XArgs {
 // ---------------------------------------------------------------------------
 // Octant spectrum layout.  Entries (comp, ix, iy, iz), 0 <= i* <= N*, are
 // stored as
-//   main  [iy][iz / S][ix][comp][iz % S]   for iz <  Nz   (groups of S = 64 B)
-//   nyq   [iy][ix][comp]                   for iz == Nz   (after main)
-// so that the entries one x-pass block needs (one iy, S consecutive iz, all
-// ix and components) are one contiguous run.  S = 8 (complex64) or 4
-// (complex128); the last iz group is zero padded when S does not divide Nz.
+//   [iy][iz / S][ix][comp][iz % S]          for 0 <= iz <= Nz
+// in groups of S = 64 bytes of iz (S = 8 complex64 or 4 complex128), so that
+// the entries one x-pass block needs (one iy, S consecutive iz, all ix and
+// components) are one contiguous run, and every column, iz = Nz included,
+// has the same strides.  G = Nz / S + 1 groups per iy; slots past Nz in the
+// last group are padding (never read).
 // ---------------------------------------------------------------------------
 struct KhatLayout {
   int nx1, ny1, nz, K, S, G;
-  E3_HD long long main_elems() const { return (long long)ny1 * G * nx1 * K * S; }
-  E3_HD long long total() const { return main_elems() + (long long)ny1 * nx1 * K; }
+  E3_HD long long row_elems() const { return (long long)G * nx1 * K * S; }
+  E3_HD long long total() const { return (long long)ny1 * row_elems(); }
   E3_HD long long index(int comp, int ix, int iy, int iz) const {
-    if (iz < nz) return ((((long long)iy * G + iz / S) * nx1 + ix) * K + comp) * S + iz % S;
-    return main_elems() + ((long long)iy * nx1 + ix) * K + comp;
+    return ((((long long)iy * G + iz / S) * nx1 + ix) * K + comp) * S + iz % S;
   }
 };
 
@@ -327,7 +327,7 @@ E3_HD KhatLayout khat_layout(const int N[3], int ncomp, int elem_bytes) {
   l.nz = N[2];
   l.K = ncomp == 3 ? 6 : 1;
   l.S = 64 / elem_bytes;
-  l.G = (N[2] + l.S - 1) / l.S;
+  l.G = N[2] / l.S + 1;
   return l;
 }
 
@@ -498,7 +498,15 @@ E3_HD void octant_point(const KBArgs& a, long long idx) {
   const int iy = (int)((r / nz1) % ny1);
   const int ix = (int)(r / (nz1 * ny1));
   const long long D = (long long)a.L[0] * a.L[1] * a.L[2];
-  const cpx<double> v = full[c * D + ((long long)ix * a.L[1] + iy) * a.L[2] + iz];
+  cpx<double> v = full[c * D + ((long long)ix * a.L[1] + iy) * a.L[2] + iz];
+  // the off-diagonal components xy, xz, yz are odd along both their axes, so
+  // their spectrum vanishes on those axes' index-N planes: stored as exact
+  // zeros (the FFT leaves rounding residue there), which lets the x pass
+  // take either sign for kx = Nx.
+  if (a.kind == 0 && c >= 3) {
+    const bool ox = c != 5 && ix == a.N[0], oy = c != 4 && iy == a.N[1], oz = c != 3 && iz == a.N[2];
+    if (ox || oy || oz) v = mk(0.0, 0.0);
+  }
   const KhatLayout lay = khat_layout(a.N, a.kind == 0 ? 3 : 1, (int)sizeof(cpx<T>));
   oct[lay.index(c, ix, iy, iz)] = mk((T)(v.re * a.scale), (T)(v.im * a.scale));
 }

@@ -54,6 +54,22 @@ template <typename T> E3_HD cpx<T> operator*(T s, cpx<T> a) { return mk(s * a.re
 template <typename T> E3_HD cpx<T> operator*(cpx<T> a, T s) { return mk(a.re * s, a.im * s); }
 template <typename T> E3_HD cpx<T> conj(cpx<T> a) { return mk(a.re, -a.im); }
 template <typename T> E3_HD cpx<T> czero() { return mk(T(0), T(0)); }
+
+// fused multiply-add of the real type
+E3_HD float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+E3_HD double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// k x and acc + k x with k conjugated when CJ, as 2 + 2 fused operations
+template <bool CJ, typename T>
+E3_HD cpx<T> cmul_f(cpx<T> k, cpx<T> x) {
+  const T ki = CJ ? -k.im : k.im;
+  return mk(fmaT(k.re, x.re, -(ki * x.im)), fmaT(k.re, x.im, ki * x.re));
+}
+template <bool CJ, typename T>
+E3_HD void cmac_f(cpx<T>& acc, cpx<T> k, cpx<T> x) {
+  const T ki = CJ ? -k.im : k.im;
+  acc.re = fmaT(-ki, x.im, fmaT(k.re, x.re, acc.re));
+  acc.im = fmaT(ki, x.re, fmaT(k.re, x.im, acc.im));
+}
 // conj(a) * b
 template <typename T> E3_HD cpx<T> cmulc(cpx<T> a, cpx<T> b) {
   return mk(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re);

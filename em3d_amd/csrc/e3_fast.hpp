@@ -428,15 +428,28 @@ struct FastX {
   E3_HD void stage_tables(int tid) const {
     for (int t = tid; t < L; t += NT) lds[SEG + t] = ((const cpx<T>*)a.tw)[t];
     if (STAGE) {
-      const int Nn[3] = {a.N[0], a.N[1], a.N[2]};
-      const KhatLayout lay = khat_layout(Nn, C, (int)sizeof(cpx<T>));
-      const cpx<T>* kg = (const cpx<T>*)a.khat + (((long long)pr * lay.G + (pos0 >> 1) / S) * NX1) * K * S;
-      for (int t = tid; t < KS_MAIN; t += NT) kst()[t] = kg[t];
+      // LDS copy [ix][comp][S + 1]: slots 0 .. S-1 the tile's column group,
+      // slot S the column iz = Nz (tile 0 only)
+      const cpx<T>* row = krow();
+      const cpx<T>* kg = row + (long long)((pos0 >> 1) / S) * NX1 * K * S;
+      for (int t = tid; t < KS_MAIN; t += NT) kst()[(t / S) * (S + 1) + t % S] = kg[t];
       if (pos0 == 0) {
-        const cpx<T>* kn = (const cpx<T>*)a.khat + lay.main_elems() + (long long)pr * NX1 * K;
-        for (int t = tid; t < NX1 * K; t += NT) kst()[KS_MAIN + t] = kn[t];
+        const int nz = a.N[2];
+        const cpx<T>* kn = row + (long long)(nz / S) * NX1 * K * S + nz % S;
+        for (int t = tid; t < NX1 * K; t += NT) kst()[t * (S + 1) + S] = kn[(long long)t * S];
       }
     }
+  }
+  // sign of component c on this lane's column: the spectrum is applied as
+  // diag(sx, sy, sz) K diag(sx, sy, sz) with K the octant entries; sy (row
+  // Ly - p) and sz (odd positions >= 3, kz > Nz) do not depend on kx, so they
+  // are applied to the rows of components 1 and 2 as they are loaded and
+  // stored, and only sx remains in the product.
+  E3_HD T comp_sign(int c, const Lane& q) const {
+    if (C != 3 || c == 0) return T(1);
+    if (c == 1) return q.h ? T(-1) : T(1);
+    const int pos = pos0 + q.col;
+    return ((pos & 1) && pos > 1) ? T(-1) : T(1);
   }
   // ---- forward stage A (Ns = 1): global rows < L/2 -> regs -> LDS ----
   // stage-A butterfly jj = q.j + bb NTC reads rows jj + m L/RA, live for m < RA/2
@@ -449,10 +462,11 @@ struct FastX {
     for (int bb = 0; bb < BA; ++bb) {
       const int jj = q.j + bb * NTC;
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
+      const T sg = comp_sign(c, q);
       cpx<T> v[RA];
 #pragma unroll
       for (int m = 0; m < RA; ++m)
-        v[m] = m < RA / 2 ? *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) : czero<T>();
+        v[m] = m < RA / 2 ? *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) * sg : czero<T>();
       fft_reg<T, RA, false>(v);
 #pragma unroll
       for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
@@ -544,58 +558,69 @@ struct FastX {
     fft_reg<T, RC, false>(v);
   }
   // ---- spectral product at kx = j + m NTC ----
-  // block-uniform bases of this block's octant run (main group, Nyquist column)
-  E3_HD const cpx<T>* krun() const {
+  // The block's octant row (|ky| = p): entry (ix, comp) of column iz at element
+  // ((iz / S) NX1 + ix) K S + comp S + iz % S.
+  E3_HD const cpx<T>* krow() const {
     const int Nn[3] = {a.N[0], a.N[1], a.N[2]};
     const KhatLayout lay = khat_layout(Nn, C, (int)sizeof(cpx<T>));
-    return (const cpx<T>*)a.khat + (((long long)pr * lay.G + (pos0 >> 1) / S) * NX1) * K * S;
+    return (const cpx<T>*)a.khat + (long long)pr * lay.row_elems();
   }
-  E3_HD const cpx<T>* knyq() const {
-    const int Nn[3] = {a.N[0], a.N[1], a.N[2]};
-    const KhatLayout lay = khat_layout(Nn, C, (int)sizeof(cpx<T>));
-    return (const cpx<T>*)a.khat + lay.main_elems() + (long long)pr * NX1 * K;
-  }
-  E3_HD cpx<T> kentry(const cpx<T>* run, const cpx<T>* nq, int comp, int ix, bool nyq, int slot) const {
-    if (STAGE) return nyq ? kst()[KS_MAIN + ix * K + comp] : kst()[(ix * K + comp) * S + slot];
-    return nyq ? *at_offset(nq, (unsigned)((ix * K + comp) * sizeof(cpx<T>)))
-               : *at_offset(run, (unsigned)(((ix * K + comp) * S + slot) * sizeof(cpx<T>)));
+  static constexpr int KSTR = STAGE ? S + 1 : S;   // element stride between components
+  E3_HD cpx<T> kload(const cpx<T>* base, int off) const {
+    if constexpr (STAGE) return base[off];
+    return *at_offset(base, (unsigned)off * (unsigned)sizeof(cpx<T>));
   }
   E3_HD void multiply(int tid, int h, St& s) const {
     Lane q = lane(tid, h);
-    E3_OPAQUE(q.j);   // the 6 RC octant addresses are formed here, not held across the row
+    E3_OPAQUE(q.j);   // the octant addresses are formed here, not held across the unit
     E3_OPAQUE(q.col);
     const int pos = pos0 + q.col;
-    const bool nyq = pos == 1;                          // kz = Nz
-    const int slot = (pos >> 1) % S;
-    const T sy = q.h ? T(-1) : T(1);                    // row Ly - p is the mirrored one
-    const T sz = ((pos & 1) && pos > 1) ? T(-1) : T(1); // odd positions >= 3 hold kz > Nz
-    const bool adj = a.adj != 0;
-    const cpx<T>* run = krun();
-    const cpx<T>* nq = knyq();
+    const int iz = pos == 1 ? a.N[2] : (pos >> 1);    // octant column (position 1 holds kz = Nz)
+    int t0;                                           // element of (ix = 0, comp = 0) of the column
+    if constexpr (STAGE) t0 = pos == 1 ? S : iz % S;
+    else t0 = (iz / S) * NX1 * K * S + iz % S;
+    const cpx<T>* base = STAGE ? (const cpx<T>*)kst() : krow();
+    product(q, base, t0, a.adj ? T(-1) : T(1), s);
+  }
+  // kx = j + m NTC folds to ix = L - kx for m >= RC/2 (kx > L/2 there, except
+  // kx = L/2 itself, where both folds give ix = L/2 and the odd entries xy, xz
+  // are stored as zeros): the sign sx is a compile-time constant per m.
+  // sj = -1 applies the adjoint (conjugated entries)
+  E3_HD void product(const Lane& q, const cpx<T>* base, int t0, T sj, St& s) const {
 #pragma unroll
     for (int m = 0; m < RC; ++m) {
       if (m % kMulGroup == 0) E3_SCHED_FENCE();        // bound the reads in flight
       const int kx = q.j + m * NTC;
-      const bool fx = kx > L / 2;
+      const bool fx = (RC % 2 == 0) ? (m >= RC / 2) : (kx > L / 2);
       const int ix = fx ? L - kx : kx;
-      if (C == 1) {
-        cpx<T> k = kentry(run, nq, 0, ix, nyq, slot);
-        if (adj) k = conj(k);
-        s.X[0][m] = k * s.X[0][m];
+      const int off = t0 + ix * K * KSTR;
+      if constexpr (C == 1) {
+        const cpx<T> k = kload(base, off);
+        s.X[0][m] = cmul_f<false>(mk(k.re, k.im * sj), s.X[0][m]);
       } else {
         cpx<T> k[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-          k[i] = kentry(run, nq, i, ix, nyq, slot);
-          if (adj) k[i] = conj(k[i]);
+          k[i] = kload(base, off + i * KSTR);
+          k[i].im *= sj;
         }
         const T sx = fx ? T(-1) : T(1);
-        const cpx<T> kxy = (sx * sy) * k[3], kxz = (sx * sz) * k[4], kyz = (sy * sz) * k[5];
-        const int c1 = C > 1 ? 1 : 0, c2 = C > 2 ? 2 : 0;
-        const cpx<T> f0 = s.X[0][m], f1 = s.X[c1][m], f2 = s.X[c2][m];
-        s.X[0][m] = k[0] * f0 + kxy * f1 + kxz * f2;
-        s.X[c1][m] = kxy * f0 + k[1] * f1 + kyz * f2;
-        s.X[c2][m] = kxz * f0 + kyz * f1 + k[2] * f2;
+        // X1 = sy f1, X2 = sz f2 (signs applied at load); outputs 1, 2 carry
+        // sy, sz, removed at store
+        const cpx<T> f0 = s.X[0][m], g1 = s.X[1][m], g2 = s.X[2][m];
+        const cpx<T> h0 = f0 * sx, h1 = g1 * sx, h2 = g2 * sx;
+        cpx<T> o0 = cmul_f<false>(k[0], f0);
+        cmac_f<false>(o0, k[3], h1);
+        cmac_f<false>(o0, k[4], h2);
+        cpx<T> o1 = cmul_f<false>(k[3], h0);
+        cmac_f<false>(o1, k[1], g1);
+        cmac_f<false>(o1, k[5], g2);
+        cpx<T> o2 = cmul_f<false>(k[4], h0);
+        cmac_f<false>(o2, k[5], g1);
+        cmac_f<false>(o2, k[2], g2);
+        s.X[0][m] = o0;
+        s.X[1][m] = o1;
+        s.X[2][m] = o2;
       }
     }
   }
@@ -626,8 +651,9 @@ struct FastX {
       }
       fft_reg<T, RA, true>(v);
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
+      const T sg = comp_sign(c, q);
 #pragma unroll
-      for (int m = 0; m < RA / 2; ++m) *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) = v[m];
+      for (int m = 0; m < RA / 2; ++m) *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) = v[m] * sg;
     }
   }
 };

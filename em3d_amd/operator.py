@@ -94,20 +94,17 @@ class HipPlan:
 def khat_storage_index(N, ncomp: int, elem_bytes: int):
     """Storage index of every logical octant entry (comp, ix, iy, iz) as an int64
     array of shape (K, Nx+1, Ny+1, Nz+1).  The library stores the octant as
-    main[iy][iz // S][ix][comp][iz % S] for iz < Nz (S = 64 bytes of entries)
-    followed by nyq[iy][ix][comp] for iz = Nz."""
+    [iy][iz // S][ix][comp][iz % S] for 0 <= iz <= Nz (S = 64 bytes of
+    entries, G = Nz // S + 1 groups per iy)."""
     K = 6 if ncomp == 3 else 1
     S = 64 // elem_bytes
-    nx1, ny1, nz = N[0] + 1, N[1] + 1, N[2]
-    G = (nz + S - 1) // S
+    nx1, nz = N[0] + 1, N[2]
+    G = nz // S + 1
     c = np.arange(K)[:, None, None, None]
     ix = np.arange(nx1)[None, :, None, None]
-    iy = np.arange(ny1)[None, None, :, None]
+    iy = np.arange(N[1] + 1)[None, None, :, None]
     iz = np.arange(nz + 1)[None, None, None, :]
-    main = ((((iy * G + iz // S) * nx1 + ix) * K + c) * S + iz % S)
-    main_elems = ny1 * G * nx1 * K * S
-    nyq = main_elems + (iy * nx1 + ix) * K + c
-    return np.where(iz < nz, main, nyq).astype(np.int64)
+    return ((((iy * G + iz // S) * nx1 + ix) * K + c) * S + iz % S).astype(np.int64)
 
 
 def octant_entries(khat_flat, N, ncomp: int):
