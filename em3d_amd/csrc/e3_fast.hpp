@@ -362,6 +362,13 @@ struct XR {
   static constexpr int A = A_, B = B_, C = C_;
 };
 
+#ifndef E3_X_PINGPONG
+#define E3_X_PINGPONG 1
+#endif
+#ifndef E3_X_MULGROUP
+#define E3_X_MULGROUP 2
+#endif
+
 template <typename T, int L, int C, int TC_ = 128 / (int)sizeof(cpx<T>), bool STAGE_ = true, int WAVES_ = 2,
           class RX = XRadix<L>>
 struct FastX {
@@ -379,10 +386,14 @@ struct FastX {
   // the block's octant run: staged once in LDS (STAGE_) or read through the caches
   static constexpr int KGRP = TC / 2;                  // octant columns of the block
   static constexpr int KS_MAIN = NX1 * K * S, KS = NX1 * K * (S + 1);
-  static constexpr bool STAGE = STAGE_ && (size_t)(L * (TC + 1) + L + KS) * sizeof(cpx<T>) <= 160 * 1024;
+  // two row tiles (ping-pong: every LDS stage writes the other tile) when a
+  // block then still fits twice into a CU's 160 KB; else one tile, in place
+  static constexpr bool PP = E3_X_PINGPONG && (size_t)(2 * SEG + L + (STAGE_ ? KS : 0)) * sizeof(cpx<T>) <= 80 * 1024;
+  static constexpr int NSEG = PP ? 2 : 1;
+  static constexpr bool STAGE = STAGE_ && (size_t)(NSEG * SEG + L + KS) * sizeof(cpx<T>) <= 160 * 1024;
   static constexpr int MIN_WAVES_PER_EU = WAVES_;
-  static constexpr int LDS = SEG + L + (STAGE ? KS : 0);
-  static constexpr int kMulGroup = 2;                  // spectral products scheduled together
+  static constexpr int LDS = NSEG * SEG + L + (STAGE ? KS : 0);
+  static constexpr int kMulGroup = E3_X_MULGROUP;      // spectral products scheduled together
   static_assert(RA * (RB > 1 ? RB : 1) * RC == L, "radix plan");
   static_assert(RC % RA == 0 && (RB == 1 || RC % RB == 0), "stage radices must divide the last one");
   struct St {
@@ -419,14 +430,14 @@ struct FastX {
     return q;
   }
   E3_HD cpx<T>* plane(int c) const { return (cpx<T>*)a.data + (size_t)b * a.bs + (size_t)c * a.cs; }
-  E3_HD cpx<T>* seg(int) const { return lds; }
-  E3_HD const cpx<T>* twl() const { return lds + SEG; }
-  E3_HD cpx<T>* kst() const { return lds + SEG + L; }
+  E3_HD cpx<T>* seg(int buf) const { return lds + buf * SEG; }
+  E3_HD const cpx<T>* twl() const { return lds + NSEG * SEG; }
+  E3_HD cpx<T>* kst() const { return lds + NSEG * SEG + L; }
   E3_HD cpx<T>& cell(cpx<T>* t, int idx, int col) const { return t[idx * TCP + col]; }
 
   // ---- loads: twiddles, the block's octant run, all components' inputs ----
   E3_HD void stage_tables(int tid) const {
-    for (int t = tid; t < L; t += NT) lds[SEG + t] = ((const cpx<T>*)a.tw)[t];
+    for (int t = tid; t < L; t += NT) lds[NSEG * SEG + t] = ((const cpx<T>*)a.tw)[t];
     if (STAGE) {
       // LDS copy [ix][comp][S + 1]: slots 0 .. S-1 the tile's column group,
       // slot S the column iz = Nz (tile 0 only)
@@ -453,10 +464,10 @@ struct FastX {
   }
   // ---- forward stage A (Ns = 1): global rows < L/2 -> regs -> LDS ----
   // stage-A butterfly jj = q.j + bb NTC reads rows jj + m L/RA, live for m < RA/2
-  E3_HD void fwd_a(int c, int tid, int h, St& s) const {
+  E3_HD void fwd_a(int c, int tid, int h, St& s, int buf) const {
     Lane q = lane(tid, h);
     E3_OPAQUE(q.j);   // row addresses are formed per phase, not held across the row
-    cpx<T>* t = seg(q.h);
+    cpx<T>* t = seg(buf);
     const cpx<T>* d = plane(c);
 #pragma unroll
     for (int bb = 0; bb < BA; ++bb) {
@@ -473,10 +484,10 @@ struct FastX {
     }
   }
   // ---- middle stage (radix RB): read | barrier | write, in place ----
-  E3_HD void mid_read(int tid, int h, St& s) const {
+  E3_HD void mid_read(int tid, int h, St& s, int buf) const {
     if constexpr (RB > 1) {
       const Lane q = lane(tid, h);
-      const cpx<T>* t = seg(q.h);
+      const cpx<T>* t = seg(buf);
 #pragma unroll
       for (int bb = 0; bb < BB; ++bb) {
         const int jj = q.j + bb * NTC;
@@ -486,10 +497,10 @@ struct FastX {
     }
   }
   template <bool INV>
-  E3_HD void mid_write(int tid, int h, St& s, int ns) const {
+  E3_HD void mid_write(int tid, int h, St& s, int ns, int buf) const {
     if constexpr (RB > 1) {
       const Lane q = lane(tid, h);
-      cpx<T>* t = seg(q.h);
+      cpx<T>* t = seg(buf);
       const cpx<T>* tw = twl();
 #pragma unroll
       for (int bb = 0; bb < BB; ++bb) {
@@ -510,44 +521,96 @@ struct FastX {
   // compile-time instances, the row index is not, so nothing is carried from
   // one row to the next in registers.
   static constexpr int MID = RB > 1 ? 1 : 0;
-  static constexpr int PPC = 2 + 2 * MID;              // phases per component and direction
-  static constexpr int NPH = 2 * C * PPC;              // phases per row
+  static constexpr int PPC = 2 + 2 * MID;              // phases per component and direction (one tile)
+  static constexpr int NPH = PP ? (MID ? 4 * C + 1 : 2 * C + 1) : 2 * C * PPC;
   E3_HD int rows() const { return two ? 2 : 1; }
   template <int I>
   E3_HD void phase(int tid, int h, St& s) const {
+    if constexpr (I == 0) {
+      if (h == 0) stage_tables(tid);
+    }
+    if constexpr (PP) phase_pp<I>(tid, h, s);
+    else phase_one<I>(tid, h, s);
+  }
+  // Two tiles.  A stage that reads the tile written in the previous phase
+  // writes the other one, so a middle stage is one phase and the last stage
+  // of a component shares its phase with the first stage of the next one.
+  // With a middle stage (MID): stage A and inverse stage C write tile 0, the
+  // middle stage reads 0 and writes 1, stage C and inverse stage A read 1:
+  //   A0 | M0 | C0 A1 | M1 | C1 A2 | M2 | C2 x IC0 | IM0 | IA0 IC1 | IM1 | IA1 IC2 | IM2 | IA2
+  // Without: the k-th tile write of the row goes to tile k % 2:
+  //   A0 | C0 A1 | C1 A2 | C2 x IC0 | IA0 IC1 | IA1 IC2 | IA2
+  template <int I>
+  E3_HD void phase_pp(int tid, int h, St& s) const {
+    if constexpr (I == 0) {
+      fwd_a(0, tid, h, s, 0);
+    } else if constexpr (MID) {
+      constexpr bool fwd = I <= 2 * C;
+      constexpr int J = fwd ? I - 1 : I - 2 * C - 1;
+      constexpr int c = J / 2;
+      if constexpr (J % 2 == 0) {
+        mid_read(tid, h, s, 0);
+        mid_write<!fwd>(tid, h, s, fwd ? RA : RC, 1);
+      } else if constexpr (fwd) {
+        fwd_c(c, tid, h, s, 1);
+        if constexpr (c + 1 < C) {
+          fwd_a(c + 1, tid, h, s, 0);
+        } else {
+          multiply(tid, h, s);
+          inv_c(0, tid, h, s, 0);
+        }
+      } else {
+        inv_a(c, tid, h, 1);
+        if constexpr (c + 1 < C) inv_c(c + 1, tid, h, s, 0);
+      }
+    } else if constexpr (I <= C) {
+      constexpr int c = I - 1;
+      fwd_c(c, tid, h, s, c % 2);
+      if constexpr (c + 1 < C) {
+        fwd_a(c + 1, tid, h, s, (c + 1) % 2);
+      } else {
+        multiply(tid, h, s);
+        inv_c(0, tid, h, s, C % 2);
+      }
+    } else {
+      constexpr int c = I - C - 1;
+      inv_a(c, tid, h, (C + c) % 2);
+      if constexpr (c + 1 < C) inv_c(c + 1, tid, h, s, (C + c + 1) % 2);
+    }
+  }
+  // One tile, stages in place (the middle stage reads, then writes after a barrier).
+  template <int I>
+  E3_HD void phase_one(int tid, int h, St& s) const {
     constexpr bool fwd = I < C * PPC;
     constexpr int ii = fwd ? I : I - C * PPC;
     constexpr int c = ii / PPC, r = ii - c * PPC;
     if constexpr (fwd) {
       if constexpr (r == 0) {
-        if constexpr (c == 0) {
-          if (h == 0) stage_tables(tid);
-        }
-        fwd_a(c, tid, h, s);
+        fwd_a(c, tid, h, s, 0);
       } else if constexpr (MID && r == 1) {
-        mid_read(tid, h, s);
+        mid_read(tid, h, s, 0);
       } else if constexpr (MID && r == 2) {
-        mid_write<false>(tid, h, s, RA);
+        mid_write<false>(tid, h, s, RA, 0);
       } else {
-        fwd_c(c, tid, h, s);
+        fwd_c(c, tid, h, s, 0);
       }
     } else {
       if constexpr (r == 0) {
         if constexpr (c == 0) multiply(tid, h, s);
-        inv_c(c, tid, h, s);
+        inv_c(c, tid, h, s, 0);
       } else if constexpr (MID && r == 1) {
-        mid_read(tid, h, s);
+        mid_read(tid, h, s, 0);
       } else if constexpr (MID && r == 2) {
-        mid_write<true>(tid, h, s, RC);
+        mid_write<true>(tid, h, s, RC, 0);
       } else {
-        inv_a(c, tid, h);
+        inv_a(c, tid, h, 0);
       }
     }
   }
   // ---- forward last stage (radix RC, Ns = L/RC): LDS -> X regs ----
-  E3_HD void fwd_c(int c, int tid, int h, St& s) const {
+  E3_HD void fwd_c(int c, int tid, int h, St& s, int buf) const {
     const Lane q = lane(tid, h);
-    const cpx<T>* t = seg(q.h);
+    const cpx<T>* t = seg(buf);
     const cpx<T>* tw = twl();
     cpx<T>* v = s.X[c];
 #pragma unroll
@@ -625,19 +688,19 @@ struct FastX {
     }
   }
   // ---- inverse first stage (radix RC, Ns = 1) from X regs -> LDS ----
-  E3_HD void inv_c(int c, int tid, int h, St& s) const {
+  E3_HD void inv_c(int c, int tid, int h, St& s, int buf) const {
     const Lane q = lane(tid, h);
-    cpx<T>* t = seg(q.h);
+    cpx<T>* t = seg(buf);
     cpx<T>* v = s.X[c];
     fft_reg<T, RC, true>(v);
 #pragma unroll
     for (int m = 0; m < RC; ++m) cell(t, q.j * RC + m, q.col) = v[m];
   }
   // ---- inverse last stage (radix RA, Ns = L/RA): LDS -> rows < Nx ----
-  E3_HD void inv_a(int c, int tid, int h) const {
+  E3_HD void inv_a(int c, int tid, int h, int buf) const {
     Lane q = lane(tid, h);
     E3_OPAQUE(q.j);
-    const cpx<T>* t = seg(q.h);
+    const cpx<T>* t = seg(buf);
     const cpx<T>* tw = twl();
     cpx<T>* d = plane(c);
 #pragma unroll
