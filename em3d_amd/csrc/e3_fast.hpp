@@ -365,6 +365,9 @@ struct XR {
 #ifndef E3_X_PINGPONG
 #define E3_X_PINGPONG 1
 #endif
+#ifndef E3_X_PAD
+#define E3_X_PAD 1   // row-tile padding (elements) against LDS bank conflicts
+#endif
 #ifndef E3_X_MULGROUP
 #define E3_X_MULGROUP 2
 #endif
@@ -377,7 +380,7 @@ struct FastX {
   static constexpr int RA = RX::A, RB = RX::B, RC = RX::C;
   static constexpr int NTC = L / RC;                   // threads per column
   static constexpr int S = 64 / (int)sizeof(cpx<T>);   // octant group: 64 bytes of iz
-  static constexpr int TC = TC_, TCP = TC + 1;        // columns per block, TC divides 2S
+  static constexpr int TC = TC_, TCP = TC + E3_X_PAD;  // columns per block, TC divides 2S
   static constexpr int NT = TC * NTC;                  // threads of a block (one row at a time)
   static constexpr int BA = RC / RA, BB = RB > 1 ? RC / RB : 0;   // butterflies per thread
   static constexpr int K = C == 3 ? 6 : 1;
@@ -756,20 +759,35 @@ struct FastZF {
   cpx<T>* lds;
   E3_HD FastZF(const ZArgs& args, long long blk, int bidx, cpx<T>* l) : a(args), p0(blk * P), b(bidx), lds(l) {}
 
-  E3_HD void stage_a(int c, int tid) const {
-    const int j = tid % R2, pl = tid / R2;
+  // stage A in two halves, so a kernel can issue the loads of the next
+  // component before transforming the current one
+  E3_HD void load_a(int c, int tid, cpx<T>* v) const {
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
+    E3_OPAQUE(pl);
     const long long p = p0 + pl;
     const bool ok = p < a.P;
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs;
-    cpx<T> v[R1];
 #pragma unroll
     for (int m = 0; m < R1; ++m) v[m] = (m < MA && ok) ? contrast_row<T>(a, in, c, p, j + m * R2) : czero<T>();
+  }
+  E3_HD void fft_a(int tid, cpx<T>* v) const {
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
+    E3_OPAQUE(pl);
     fft_reg<T, R1, false>(v);
 #pragma unroll
     for (int k = 0; k < R1; ++k) lds[pl * PS + k * R2P + j] = v[k];
   }
+  E3_HD void stage_a(int c, int tid) const {
+    cpx<T> v[R1];
+    load_a(c, tid, v);
+    fft_a(tid, v);
+  }
   E3_HD void stage_b(int c, int tid) const {
-    const int j = tid % R2, pl = tid / R2;
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
+    E3_OPAQUE(pl);
     const long long p = p0 + pl;
     const cpx<T>* tw = (const cpx<T>*)a.tw;
     cpx<T>* out = (cpx<T>*)a.out + (size_t)b * a.out_bs + (size_t)c * a.out_cs + (size_t)p * a.out_ps;
@@ -792,20 +810,23 @@ struct FastZF {
 };
 
 // ---------------------------------------------------------------------------
-// z pass, inverse with the epilogue y = u - t or y = v - chi^H t; keeps the
-// first L/2 outputs of all C components in registers until the epilogue.
+// z pass, inverse with the epilogue y = u - t or y = v - chi^H t.  HOLD
+// (the adjoint with a full 3x3 contrast, whose chi^H mixes components): the
+// first L/2 outputs of all C components stay in registers until one
+// epilogue; otherwise each component's epilogue follows its transform.
 // ---------------------------------------------------------------------------
-template <typename T, int L, int R1_, int P_, int C>
+template <typename T, int L, int R1_, int P_, int C, bool HOLD_ = true>
 struct FastZI {
   using Real = T;
   static constexpr int NCOMP = C;
+  static constexpr bool HOLD = HOLD_ && C > 1;
   static constexpr int R1 = R1_, R2 = L / R1_, P = P_, NT = P * R2, R2P = R2 + 1;
   static constexpr int PS = R1 * R2P;
   static constexpr int NB = R1 / R2;
   static constexpr int LDS = P * PS + L;       // pencil tiles + twiddle table
   static constexpr int MH = R2 / 2;
   struct St {
-    cpx<T> y[C][NB][MH];
+    cpx<T> y[HOLD ? C : 1][NB][MH];
   };
   const ZArgs& a;
   long long p0;
@@ -813,25 +834,38 @@ struct FastZI {
   cpx<T>* lds;
   E3_HD FastZI(const ZArgs& args, long long blk, int bidx, cpx<T>* l) : a(args), p0(blk * P), b(bidx), lds(l) {}
 
-  E3_HD void stage_a(int c, int tid) const {
-    if (c == 0)
-      for (int t = tid; t < L; t += NT) lds[P * PS + t] = ((const cpx<T>*)a.tw)[t];
-    const int j = tid % R2, pl = tid / R2;
+  E3_HD void load_a(int c, int tid, cpx<T>* v) const {
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
+    E3_OPAQUE(pl);
     const long long p = p0 + pl;
     const bool ok = p < a.P;
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs + (size_t)c * a.in_cs + (size_t)p * a.in_ps;
-    cpx<T> v[R1];
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
       const int n = j + m * R2;
       v[m] = ok ? in[a.perm ? mirror_position(n, L / 2, L) : n] : czero<T>();
     }
+  }
+  E3_HD void fft_a(int c, int tid, cpx<T>* v) const {
+    if (c == 0)
+      for (int t = tid; t < L; t += NT) lds[P * PS + t] = ((const cpx<T>*)a.tw)[t];
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
+    E3_OPAQUE(pl);
     fft_reg<T, R1, true>(v);
 #pragma unroll
     for (int k = 0; k < R1; ++k) lds[pl * PS + k * R2P + j] = v[k];
   }
+  E3_HD void stage_a(int c, int tid) const {
+    cpx<T> v[R1];
+    load_a(c, tid, v);
+    fft_a(c, tid, v);
+  }
   E3_HD void stage_b(int c, int tid, St& s) const {
-    const int j = tid % R2, pl = tid / R2;
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
+    E3_OPAQUE(pl);
     const cpx<T>* tw = lds + P * PS;
 #pragma unroll
     for (int bb = 0; bb < NB; ++bb) {
@@ -844,11 +878,39 @@ struct FastZI {
       }
       fft_reg<T, R2, true>(v);
 #pragma unroll
-      for (int m = 0; m < MH; ++m) s.y[c][bb][m] = v[m];
+      for (int m = 0; m < MH; ++m) s.y[HOLD ? c : 0][bb][m] = v[m];
+    }
+  }
+  // epilogue of component c alone (not HOLD): y_c = u_c - t_c or u_c - conj(chi_c) t_c
+  E3_HD void epilogue_one(int c, int tid, St& s) const {
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);
+    E3_OPAQUE(pl);
+    const long long p = p0 + pl;
+    if (p >= a.P) return;
+    cpx<T>* out = (cpx<T>*)a.out + (size_t)b * a.out_bs;
+    const cpx<T>* src = (const cpx<T>*)a.u + (size_t)b * a.out_bs;
+    const cpx<T>* chi = (const cpx<T>*)a.chi;
+    const bool herm = a.epi == EPI_SUB_CHIH && a.chi_kind != CHI_NONE;
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) {
+#pragma unroll
+      for (int m = 0; m < MH; ++m) {
+        const int z = j + bb * R2 + m * R1;
+        cpx<T> t = s.y[0][bb][m];
+        if (herm) {
+          const long long cell = p * a.n_out + z;
+          t = cmulc(a.chi_kind == CHI_ISO ? chi[cell] : chi[(size_t)c * a.ncell + cell], t);
+        }
+        const size_t o = (size_t)c * a.out_cs + (size_t)p * a.out_ps + z;
+        out[o] = src[o] - t;
+      }
     }
   }
   E3_HD void epilogue(int tid, St& s) const {
-    const int j = tid % R2, pl = tid / R2;
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
+    E3_OPAQUE(pl);
     const long long p = p0 + pl;
     if (p >= a.P) return;
     cpx<T>* out = (cpx<T>*)a.out + (size_t)b * a.out_bs;
@@ -857,6 +919,7 @@ struct FastZI {
     for (int bb = 0; bb < NB; ++bb) {
 #pragma unroll
       for (int m = 0; m < MH; ++m) {
+        if (m % 2 == 0) E3_SCHED_FENCE();   // bound the contrast and field reads in flight
         const int z = j + bb * R2 + m * R1;
         cpx<T> t[C];
 #pragma unroll
@@ -998,8 +1061,12 @@ bool fast_z_var(Run& run, const ZArgs& a) {
   }
   if (a.n_in == L && a.n_out * 2 == L && a.inv == 1 && a.pro == PRO_COPY &&
       (a.epi == EPI_SUB || a.epi == EPI_SUB_CHIH)) {
-    if (a.C == 3) run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 3>>(a);
-    else run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 1>>(a);
+    if (a.C == 3 && a.epi == EPI_SUB_CHIH && a.chi_kind == CHI_FULL)
+      run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 3, true>>(a);
+    else if (a.C == 3)
+      run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 3, false>>(a);
+    else
+      run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 1, false>>(a);
     return true;
   }
   if constexpr (sizeof(T) == 8) {

@@ -102,8 +102,11 @@ struct EmuFastRun {
       for (int c = 0; c < a.C; ++c) {
         for (int t = 0; t < K::NT; ++t) k.stage_a(c, t);
         for (int t = 0; t < K::NT; ++t) k.stage_b(c, t, st[t]);
+        if constexpr (!K::HOLD)
+          for (int t = 0; t < K::NT; ++t) k.epilogue_one(c, t, st[t]);
       }
-      for (int t = 0; t < K::NT; ++t) k.epilogue(t, st[t]);
+      if constexpr (K::HOLD)
+        for (int t = 0; t < K::NT; ++t) k.epilogue(t, st[t]);
     }
   }
 };
