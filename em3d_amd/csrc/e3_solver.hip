@@ -357,11 +357,50 @@ __global__ __launch_bounds__(kRT) void k_group(const BiState* sts, int nb, Group
 }
 
 // first history entry of every RHS (relative residual of u = 0 and its action count)
-__global__ void k_hist0(const BiState* sts, const double* rel0, int nb, Hist H) {
+// Start state of every RHS from the norm partials of f (r = f - A 0 = f, A 0
+// counted as one action) and its first history entry.  One thread per RHS
+// sums the nb partials in order, as a host loop would.
+__global__ void k_init(const double* part, int nb, int B, em3d_bicgstab_params prm, BiState* sts, Hist H) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  H.h[(size_t)b * H.cap] = rel0[b];
-  H.a[(size_t)b * H.cap] = sts[b].mv;
+  if (b >= B) return;
+  double nn = 0.0;
+  for (int k = 0; k < nb; ++k) nn += part[((size_t)b * kRB + k) * 4];
+  const double rn = sqrt(nn);
+  BiState s = {};
+  s.rhs_norm = rn;
+  s.divg = prm.divergence_guard;
+  s.use_divg = prm.divergence_guard >= 0.0 ? 1 : 0;
+  s.max_act = prm.max_operator_actions;
+  s.use_max_act = prm.max_operator_actions >= 0 ? 1 : 0;
+  s.max_iter = prm.max_iter;
+  s.rho_prev = mk(1.0, 0.0);
+  s.alpha = mk(1.0, 0.0);
+  s.omega = mk(1.0, 0.0);
+  s.nhist = 1;
+  s.hist_cap = H.cap;
+  s.status = ST_RUNNING;
+  double rel0;
+  if (rn == 0.0) {
+    rel0 = 0.0;
+    s.mv = 0;
+    s.status = ST_CONVERGED;
+  } else {
+    s.tol = fmax(prm.rtol, prm.atol / rn);
+    rel0 = rn / rn;
+    s.mv = 1;
+    if (!isfinite(rel0)) s.status = ST_NONFINITE;
+    else if (rel0 < s.tol) s.status = ST_CONVERGED;
+    else if (prm.max_operator_actions >= 0 && s.mv >= prm.max_operator_actions) s.status = ST_MAX_ACTIONS;
+    else if (prm.max_iter <= 0) s.status = ST_MAX_ITER;
+  }
+  s.last_rel = rel0;
+  if (s.status != ST_RUNNING) {
+    s.done = 1;
+    s.skip_t = 1;
+  }
+  sts[b] = s;
+  H.h[(size_t)b * H.cap] = rel0;
+  H.a[(size_t)b * H.cap] = s.mv;
 }
 
 int sfail(const std::string& m) { return set_error(m); }
@@ -378,7 +417,7 @@ long long field_elems(const em3d_plan_s* plan) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// solver scratch: r, r_hat, p, v, s, t (batch fields each) | partials | states | group | rel0
+// solver scratch: r, r_hat, p, v, s, t (batch fields each) | partials | states | group | rel0 (unused)
 // | preconditioned p and s (preconditioned solves only)
 struct Work {
   void* r;
@@ -512,88 +551,44 @@ int bicgstab_impl(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
     hipLaunchKernelGGL(k_dot<float>, dim3(nb, B), dim3(kRT), 0, stream, n, (const cpx<float>*)rhs,
                        (const cpx<float>*)rhs, w.part, (const BiState*)nullptr, 0);
   if ((e = hipGetLastError()) != hipSuccess) return chk(e, "norm launch");
-  std::vector<double> hp((size_t)B * kRB * 4);
-  if ((e = hipMemcpyAsync(hp.data(), w.part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, stream)) != hipSuccess ||
-      (e = hipStreamSynchronize(stream)) != hipSuccess)
-    return chk(e, "norm readback");
-  if ((e = hipMemsetAsync(u, 0, vb, stream)) != hipSuccess) return chk(e, "hipMemsetAsync(u)");
-
-  // per-RHS start: r = f - A 0 = f (A 0 is exactly 0, counted as one action)
-  std::vector<BiState> hs((size_t)B);
-  std::vector<double> rel0((size_t)B);
-  bool any_running = false;
-  for (int b = 0; b < B; ++b) {
-    double nn = 0.0;
-    for (int k = 0; k < nb; ++k) nn += hp[((size_t)b * kRB + k) * 4];
-    const double rn = std::sqrt(nn);
-    BiState& s = hs[b];
-    s = BiState{};
-    s.rhs_norm = rn;
-    s.divg = prm->divergence_guard;
-    s.use_divg = prm->divergence_guard >= 0.0 ? 1 : 0;
-    s.max_act = prm->max_operator_actions;
-    s.use_max_act = prm->max_operator_actions >= 0 ? 1 : 0;
-    s.max_iter = prm->max_iter;
-    s.rho_prev = mk(1.0, 0.0);
-    s.alpha = mk(1.0, 0.0);
-    s.omega = mk(1.0, 0.0);
-    s.nhist = 1;
-    s.hist_cap = history_cap;
-    s.status = ST_RUNNING;
-    if (rn == 0.0) {
-      rel0[b] = 0.0;
-      s.mv = 0;
-      s.status = ST_CONVERGED;
-    } else {
-      s.tol = std::max(prm->rtol, prm->atol / rn);
-      rel0[b] = rn / rn;
-      s.mv = 1;
-      if (!std::isfinite(rel0[b])) s.status = ST_NONFINITE;
-      else if (rel0[b] < s.tol) s.status = ST_CONVERGED;
-      else if (prm->max_operator_actions >= 0 && s.mv >= prm->max_operator_actions) s.status = ST_MAX_ACTIONS;
-      else if (prm->max_iter <= 0) s.status = ST_MAX_ITER;
-    }
-    s.last_rel = rel0[b];
-    if (s.status != ST_RUNNING) {
-      s.done = 1;
-      s.skip_t = 1;
-    } else {
-      any_running = true;
-    }
-  }
-  // r = r_hat = f, p = v = 0, states, first history entries
-  if ((e = hipMemcpyAsync(w.r, rhs, vb, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
+  // u = 0, r = r_hat = f, p = v = 0, start states and first history entries, all on the stream
+  if ((e = hipMemsetAsync(u, 0, vb, stream)) != hipSuccess ||
+      (e = hipMemcpyAsync(w.r, rhs, vb, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
       (e = hipMemcpyAsync(w.rh, rhs, vb, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
-      (e = hipMemsetAsync(w.p, 0, vb, stream)) != hipSuccess || (e = hipMemsetAsync(w.v, 0, vb, stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(w.st, hs.data(), (size_t)B * sizeof(BiState), hipMemcpyHostToDevice, stream)) != hipSuccess ||
-      (e = hipMemcpyAsync(w.rel0, rel0.data(), (size_t)B * sizeof(double), hipMemcpyHostToDevice, stream)) != hipSuccess)
+      (e = hipMemsetAsync(w.p, 0, vb, stream)) != hipSuccess || (e = hipMemsetAsync(w.v, 0, vb, stream)) != hipSuccess)
     return chk(e, "init");
-  hipLaunchKernelGGL(k_hist0, dim3((B + 63) / 64), dim3(64), 0, stream, w.st, w.rel0, B, H);
-  if ((e = hipGetLastError()) != hipSuccess) return chk(e, "history launch");
+  hipLaunchKernelGGL(k_init, dim3((B + 63) / 64), dim3(64), 0, stream, (const double*)w.part, nb, B, *prm, w.st, H);
+  if ((e = hipGetLastError()) != hipSuccess) return chk(e, "init launch");
 
+  // chunks of iterations, the states read back after each (a solve that is
+  // already finished at the start runs one chunk of no-op launches)
   const int chunk = prm->chunk > 0 ? prm->chunk : 4;
-  std::vector<BiState> cur(hs);
-  if (any_running) {
-    for (int it0 = 0; it0 < prm->max_iter; it0 += chunk) {
-      for (int k = 0; k < chunk && it0 + k < prm->max_iter; ++k) {
-        int fail = 0;
-        if (prec == PREC_DOUBLE)
-          enqueue_iteration<double>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, pc, stream, &fail);
-        else
-          enqueue_iteration<float>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, pc, stream, &fail);
-        if (fail) return 1;
-        if ((e = hipGetLastError()) != hipSuccess) return chk(e, "iteration launch");
-      }
-      if ((e = hipMemcpyAsync(cur.data(), w.st, (size_t)B * sizeof(BiState), hipMemcpyDeviceToHost, stream)) !=
-              hipSuccess ||
-          (e = hipStreamSynchronize(stream)) != hipSuccess)
-        return chk(e, "state readback");
-      bool all = true;
-      for (int b = 0; b < B; ++b) all = all && cur[b].done;
-      if (all) break;
+  std::vector<BiState> cur((size_t)B);
+  bool read = false;
+  for (int it0 = 0; it0 < prm->max_iter; it0 += chunk) {
+    for (int k = 0; k < chunk && it0 + k < prm->max_iter; ++k) {
+      int fail = 0;
+      if (prec == PREC_DOUBLE)
+        enqueue_iteration<double>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, pc, stream, &fail);
+      else
+        enqueue_iteration<float>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, pc, stream, &fail);
+      if (fail) return 1;
+      if ((e = hipGetLastError()) != hipSuccess) return chk(e, "iteration launch");
     }
-  } else if ((e = hipStreamSynchronize(stream)) != hipSuccess) {
-    return chk(e, "sync");
+    if ((e = hipMemcpyAsync(cur.data(), w.st, (size_t)B * sizeof(BiState), hipMemcpyDeviceToHost, stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(stream)) != hipSuccess)
+      return chk(e, "state readback");
+    read = true;
+    bool all = true;
+    for (int b = 0; b < B; ++b) all = all && cur[b].done;
+    if (all) break;
+  }
+  if (!read) {
+    if ((e = hipMemcpyAsync(cur.data(), w.st, (size_t)B * sizeof(BiState), hipMemcpyDeviceToHost, stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(stream)) != hipSuccess)
+      return chk(e, "state readback");
   }
   for (int b = 0; b < B; ++b) {
     const BiState& s = cur[b];

@@ -335,8 +335,9 @@ class _FieldApplier:
         wb = lib.em3d_bicgstab_pc_work_bytes if precond is not None else lib.em3d_bicgstab_work_bytes
         sw = torch.empty(int(wb(plan.handle)), dtype=torch.uint8, device=be.torch_device)
         cap = int(cfg.max_iter) + 2
-        hist = torch.zeros((batch, cap), dtype=torch.float64, device=be.torch_device)
-        acts = torch.zeros((batch, cap), dtype=torch.int64, device=be.torch_device)
+        # history and action counts in one buffer (entries past history_len are never read)
+        hbuf = torch.empty((2, batch, cap), dtype=torch.float64, device=be.torch_device)
+        hist, acts = hbuf[0], hbuf[1].view(torch.int64)
         u = torch.empty_like(t)
         prm = _lib.BiCGStabParams(int(cfg.max_iter), float(cfg.rtol), float(cfg.atol),
                                   -1.0 if cfg.divergence_guard is None else float(cfg.divergence_guard),
@@ -351,7 +352,9 @@ class _FieldApplier:
         else:
             _lib.check(lib.em3d_bicgstab_pc(*args, C.byref(precond.model(batch)),
                                             _lib.stream_handle(be.torch_device)), "em3d_bicgstab_pc")
-        hist_h, acts_h = hist.cpu(), acts.cpu()
+        n_max = max(int(res[b].history_len) for b in range(batch))
+        hb = hbuf[:, :, :max(n_max, 1)].cpu()
+        hist_h, acts_h = hb[0], hb[1].contiguous().view(torch.int64)
         out = []
         for b in range(batch):
             n = int(res[b].history_len)
