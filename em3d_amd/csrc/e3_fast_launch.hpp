@@ -7,10 +7,6 @@
 
 #include "e3_fast.hpp"
 
-#ifndef E3_Z_PREFETCH
-#define E3_Z_PREFETCH 0
-#endif
-
 namespace e3 {
 
 template <class K>
@@ -58,30 +54,12 @@ __global__ __launch_bounds__(K::NT) void k_fast_zf(ZArgs a, long long nblk) {
   const long long bid = blockIdx.x;
   const int b = (int)(bid / nblk);
   K k(a, bid - (long long)b * nblk, b, lds);
-#if E3_Z_PREFETCH
-  // component c + 1 is loaded while component c is transformed and stored
-  cpx<T> v[K::R1];
-  k.load_a(0, threadIdx.x, v);
-  for (int c = 0; c < a.C; ++c) {
-    cpx<T> w[K::R1];
-    if (c + 1 < a.C) k.load_a(c + 1, threadIdx.x, w);
-    k.fft_a(threadIdx.x, v);
-    __syncthreads();
-    k.stage_b(c, threadIdx.x);
-    if (c + 1 < a.C) {
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < K::R1; ++m) v[m] = w[m];
-    }
-  }
-#else
   for (int c = 0; c < a.C; ++c) {
     k.stage_a(c, threadIdx.x);
     __syncthreads();
     k.stage_b(c, threadIdx.x);
     if (c + 1 < a.C) __syncthreads();
   }
-#endif
 }
 
 template <class K, int C>
