@@ -69,6 +69,8 @@ def _chi(kind, N, seed=3):
 
 EM_CASES = [((4, 4, 4), 1.0, "full"), ((4, 3, 2), 2.1, "full"), ((5, 6, 7), 3.0, "diag"),
             ((8, 8, 8), 4.0, "iso"), ((16, 16, 16), 4.0, "diag"), ((12, 10, 14), 8.0, "full"),
+            # power-of-two fast kernels with a full contrast (the adjoint holds all components)
+            ((32, 32, 32), 6.0, "full"),
             # doubled axes 3 * 2^k and 5 * 2^k: mixed-radix register kernels in every pass
             ((24, 40, 48), 6.0, "diag"), ((48, 24, 40), 5.0, "full"), ((40, 48, 24), 4.0, "iso")]
 
