@@ -390,8 +390,10 @@ struct FastX {
   static constexpr int KGRP = TC / 2;                  // octant columns of the block
   static constexpr int KS_MAIN = NX1 * K * S, KS = NX1 * K * (S + 1);
   // two row tiles (ping-pong: every LDS stage writes the other tile) when a
-  // block then still fits twice into a CU's 160 KB; else one tile, in place
-  static constexpr bool PP = E3_X_PINGPONG && (size_t)(2 * SEG + L + (STAGE_ ? KS : 0)) * sizeof(cpx<T>) <= 80 * 1024;
+  // block then still fits twice into a CU's 160 KB, or when it has 1024
+  // threads (16 waves) and so runs alone on its CU anyway; else one tile, in place
+  static constexpr size_t PP_BYTES = (size_t)(2 * SEG + L + (STAGE_ ? KS : 0)) * sizeof(cpx<T>);
+  static constexpr bool PP = E3_X_PINGPONG && (PP_BYTES <= 80 * 1024 || (NT >= 1024 && PP_BYTES <= 160 * 1024));
   static constexpr int NSEG = PP ? 2 : 1;
   static constexpr bool STAGE = STAGE_ && (size_t)(NSEG * SEG + L + KS) * sizeof(cpx<T>) <= 160 * 1024;
   static constexpr int MIN_WAVES_PER_EU = WAVES_;
@@ -1017,9 +1019,17 @@ bool fast_x3(Run& run, const XArgs& a) {
   return true;
 }
 
+// x-pass tile of the scalar operator: columns per block, octant run staged in LDS
+#ifndef E3_X1_TC
+#define E3_X1_TC (128 / (int)sizeof(cpx<T>))
+#endif
+#ifndef E3_X1_STAGE
+#define E3_X1_STAGE true
+#endif
+
 template <typename T, int L, class Run>
 bool fast_x_len(Run& run, const XArgs& a) {
-  using X1 = FastX<T, L, 1>;
+  using X1 = FastX<T, L, 1, E3_X1_TC, E3_X1_STAGE>;
   if (a.N[0] * 2 != L || !a.perm) return false;
   if (a.C == 3) {
     constexpr int TCd = E3_X3_TC;

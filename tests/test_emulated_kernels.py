@@ -121,7 +121,8 @@ FAST_CASES = [((8, 8, 8), (1.0, 1.0, 1.0), 4.0), ((16, 8, 4), (1.0, 0.8, 0.6), 2
               ((4, 4, 160), (0.5, 0.5, 1.2), 3.0), ((2, 192, 4), (0.4, 1.0, 0.5), 2.0),
               ((48, 4, 16), (1.0, 0.5, 0.6), 3.0), ((40, 6, 8), (1.0, 0.6, 0.4), 4.0),
               ((80, 2, 8), (1.0, 0.4, 0.3), 5.0), ((96, 3, 16), (1.0, 0.3, 0.5), 3.0),
-              ((160, 2, 8), (1.0, 0.2, 0.2), 2.0), ((192, 2, 8), (1.0, 0.2, 0.2), 2.0)]
+              ((160, 2, 8), (1.0, 0.2, 0.2), 2.0), ((192, 2, 8), (1.0, 0.2, 0.2), 2.0),
+              ((256, 2, 8), (1.0, 0.1, 0.1), 6.0)]   # 512-point x pass: 1024-thread blocks, ping-pong tiles
 
 
 @pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("single", 1e-6)])
