@@ -368,6 +368,9 @@ struct XR {
 #ifndef E3_X_PAD
 #define E3_X_PAD 1   // row-tile padding (elements) against LDS bank conflicts
 #endif
+#ifndef E3_X_PREFETCH
+#define E3_X_PREFETCH 1
+#endif
 #ifndef E3_X_MULGROUP
 #define E3_X_MULGROUP 2
 #endif
@@ -404,7 +407,10 @@ struct FastX {
   struct St {
     cpx<T> X[C][RC];            // spectrum at kx = j + m (L / RC)
     cpx<T> mid[BB > 0 ? BB : 1][RB > 1 ? RB : 1];   // middle stage values in flight
+    cpx<T> pre[BA][RA / 2];     // inputs of the next stage A, loaded one phase ahead
   };
+  // loads of stage A issued one phase before the stage (ping-pong schedule)
+  static constexpr bool PREFETCH = E3_X_PREFETCH && PP;
   const XArgs& a;
   int pr, ky0, ky1, pos0, tile, b;
   bool two;            // both rows ky0 = p and ky1 = Ly - p are live
@@ -488,6 +494,40 @@ struct FastX {
       for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
     }
   }
+  // stage A in two halves: the row loads of component c of row h into s.pre,
+  // and (a phase later) the sign, the radix-RA butterflies and the LDS write
+  E3_HD void load_a(int c, int tid, int h, St& s) const {
+    Lane q = lane(tid, h);
+    E3_OPAQUE(q.j);
+    const cpx<T>* d = plane(c);
+#pragma unroll
+    for (int bb = 0; bb < BA; ++bb) {
+      const int jj = q.j + bb * NTC;
+      const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
+#pragma unroll
+      for (int m = 0; m < RA / 2; ++m) s.pre[bb][m] = *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b);
+    }
+  }
+  E3_HD void fwd_a_pre(int c, int tid, int h, St& s, int buf) const {
+    const Lane q = lane(tid, h);
+    cpx<T>* t = seg(buf);
+    const T sg = comp_sign(c, q);
+#pragma unroll
+    for (int bb = 0; bb < BA; ++bb) {
+      const int jj = q.j + bb * NTC;
+      cpx<T> v[RA];
+#pragma unroll
+      for (int m = 0; m < RA; ++m) v[m] = m < RA / 2 ? s.pre[bb][m] * sg : czero<T>();
+      fft_reg<T, RA, false>(v);
+#pragma unroll
+      for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
+    }
+  }
+  // stage A of component c, its loads either issued a phase ago or now
+  E3_HD void stage_a(int c, int tid, int h, St& s, int buf) const {
+    if constexpr (PREFETCH) fwd_a_pre(c, tid, h, s, buf);
+    else fwd_a(c, tid, h, s, buf);
+  }
   // ---- middle stage (radix RB): read | barrier | write, in place ----
   E3_HD void mid_read(int tid, int h, St& s, int buf) const {
     if constexpr (RB > 1) {
@@ -529,6 +569,7 @@ struct FastX {
   static constexpr int PPC = 2 + 2 * MID;              // phases per component and direction (one tile)
   static constexpr int NPH = PP ? (MID ? 4 * C + 1 : 2 * C + 1) : 2 * C * PPC;
   E3_HD int rows() const { return two ? 2 : 1; }
+  int h_end = 1;       // rows() once the kernel starts the row loop (prefetch of the next row)
   template <int I>
   E3_HD void phase(int tid, int h, St& s) const {
     if constexpr (I == 0) {
@@ -545,26 +586,36 @@ struct FastX {
   //   A0 | M0 | C0 A1 | M1 | C1 A2 | M2 | C2 x IC0 | IM0 | IA0 IC1 | IM1 | IA1 IC2 | IM2 | IA2
   // Without: the k-th tile write of the row goes to tile k % 2:
   //   A0 | C0 A1 | C1 A2 | C2 x IC0 | IA0 IC1 | IA1 IC2 | IA2
+  // With PREFETCH the loads of stage A of component c + 1 are issued in the
+  // phase before the one that transforms them (the block barrier waits for
+  // LDS operations only, so they stay in flight across it), and the last
+  // phase of a row issues those of component 0 of the next row (the kernel
+  // issues them for the block's first row).
   template <int I>
   E3_HD void phase_pp(int tid, int h, St& s) const {
     if constexpr (I == 0) {
-      fwd_a(0, tid, h, s, 0);
+      stage_a(0, tid, h, s, 0);
+      if constexpr (PREFETCH && !MID && C > 1) load_a(1, tid, h, s);
     } else if constexpr (MID) {
       constexpr bool fwd = I <= 2 * C;
       constexpr int J = fwd ? I - 1 : I - 2 * C - 1;
       constexpr int c = J / 2;
       if constexpr (J % 2 == 0) {
+        if constexpr (PREFETCH && fwd && c + 1 < C) load_a(c + 1, tid, h, s);
         mid_read(tid, h, s, 0);
         mid_write<!fwd>(tid, h, s, fwd ? RA : RC, 1);
       } else if constexpr (fwd) {
         fwd_c(c, tid, h, s, 1);
         if constexpr (c + 1 < C) {
-          fwd_a(c + 1, tid, h, s, 0);
+          stage_a(c + 1, tid, h, s, 0);
         } else {
           multiply(tid, h, s);
           inv_c(0, tid, h, s, 0);
         }
       } else {
+        if constexpr (PREFETCH && c + 1 == C) {
+          if (h + 1 < h_end) load_a(0, tid, h + 1, s);
+        }
         inv_a(c, tid, h, 1);
         if constexpr (c + 1 < C) inv_c(c + 1, tid, h, s, 0);
       }
@@ -572,13 +623,17 @@ struct FastX {
       constexpr int c = I - 1;
       fwd_c(c, tid, h, s, c % 2);
       if constexpr (c + 1 < C) {
-        fwd_a(c + 1, tid, h, s, (c + 1) % 2);
+        stage_a(c + 1, tid, h, s, (c + 1) % 2);             // consumes s.pre ...
+        if constexpr (PREFETCH && c + 2 < C) load_a(c + 2, tid, h, s);   // ... before it is refilled
       } else {
         multiply(tid, h, s);
         inv_c(0, tid, h, s, C % 2);
       }
     } else {
       constexpr int c = I - C - 1;
+      if constexpr (PREFETCH && c + 1 == C) {
+        if (h + 1 < h_end) load_a(0, tid, h + 1, s);
+      }
       inv_a(c, tid, h, (C + c) % 2);
       if constexpr (c + 1 < C) inv_c(c + 1, tid, h, s, (C + c + 1) % 2);
     }
@@ -638,17 +693,24 @@ struct FastX {
     if constexpr (STAGE) return base[off];
     return *at_offset(base, (unsigned)off * (unsigned)sizeof(cpx<T>));
   }
+  // element of (ix = 0, comp = 0) of this lane's octant column
+  E3_HD int kcol(const Lane& q) const {
+    const int pos = pos0 + q.col;
+    const int iz = pos == 1 ? a.N[2] : (pos >> 1);    // octant column (position 1 holds kz = Nz)
+    if constexpr (STAGE) return pos == 1 ? S : iz % S;
+    return (iz / S) * NX1 * K * S + iz % S;
+  }
+  E3_HD int koff(const Lane& q, int t0, int m) const {
+    const int kx = q.j + m * NTC;
+    const bool fx = (RC % 2 == 0) ? (m >= RC / 2) : (kx > L / 2);
+    return t0 + (fx ? L - kx : kx) * K * KSTR;
+  }
   E3_HD void multiply(int tid, int h, St& s) const {
     Lane q = lane(tid, h);
     E3_OPAQUE(q.j);   // the octant addresses are formed here, not held across the unit
     E3_OPAQUE(q.col);
-    const int pos = pos0 + q.col;
-    const int iz = pos == 1 ? a.N[2] : (pos >> 1);    // octant column (position 1 holds kz = Nz)
-    int t0;                                           // element of (ix = 0, comp = 0) of the column
-    if constexpr (STAGE) t0 = pos == 1 ? S : iz % S;
-    else t0 = (iz / S) * NX1 * K * S + iz % S;
     const cpx<T>* base = STAGE ? (const cpx<T>*)kst() : krow();
-    product(q, base, t0, a.adj ? T(-1) : T(1), s);
+    product(q, base, kcol(q), a.adj ? T(-1) : T(1), s);
   }
   // kx = j + m NTC folds to ix = L - kx for m >= RC/2 (kx > L/2 there, except
   // kx = L/2 itself, where both folds give ix = L/2 and the odd entries xy, xz
@@ -658,10 +720,8 @@ struct FastX {
 #pragma unroll
     for (int m = 0; m < RC; ++m) {
       if (m % kMulGroup == 0) E3_SCHED_FENCE();        // bound the reads in flight
-      const int kx = q.j + m * NTC;
-      const bool fx = (RC % 2 == 0) ? (m >= RC / 2) : (kx > L / 2);
-      const int ix = fx ? L - kx : kx;
-      const int off = t0 + ix * K * KSTR;
+      const bool fx = (RC % 2 == 0) ? (m >= RC / 2) : (q.j + m * NTC > L / 2);
+      const int off = koff(q, t0, m);
       if constexpr (C == 1) {
         const cpx<T> k = kload(base, off);
         s.X[0][m] = cmul_f<false>(mk(k.re, k.im * sj), s.X[0][m]);

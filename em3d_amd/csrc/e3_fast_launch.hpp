@@ -39,6 +39,8 @@ __global__ __launch_bounds__(K::NT, K::MIN_WAVES_PER_EU) void k_fast_x(XArgs a, 
   K k(a, bid - (long long)b * ncb, b, lds);
   typename K::St st;
   const int nrow = k.rows();
+  k.h_end = nrow;
+  if constexpr (K::PREFETCH) k.load_a(0, threadIdx.x, 0, st);
 #pragma unroll 1
   for (int h = 0; h < nrow; ++h) {
     if (h) __syncthreads();

@@ -64,6 +64,9 @@ struct EmuFastRun {
       const int b = (int)(bid / ncb);
       K k(a, bid - (long long)b * ncb, b, (cpx<T>*)mem.data());
       std::vector<typename K::St> st(K::NT);
+      k.h_end = k.rows();
+      if constexpr (K::PREFETCH)
+        for (int t = 0; t < K::NT; ++t) k.load_a(0, t, 0, st[t]);
       for (int h = 0; h < k.rows(); ++h) phases(k, h, st, std::make_integer_sequence<int, K::NPH>{});
     }
   }
