@@ -15,9 +15,11 @@ experiments/stationary_validation.py:172-180) and the update ``x += d``:
     until |r| / |f| < max(rtol, atol / |f|)
 
 With a ``preconditioner`` (a trained FNO) the inner solves are the
-right-preconditioned device BiCGStab of ``FlexibleBiCGStab``; an inner solve
-that does not converge within ``preconditioned_max_iter`` iterations is
-repeated without the preconditioner.
+right-preconditioned device BiCGStab of ``FlexibleBiCGStab``.  An inner solve
+that does not converge within ``preconditioned_max_iter`` iterations ends the
+use of the preconditioner: its partial correction is kept when the FP64
+residual confirms that it helped (else undone), and the remaining outer steps
+run plain inner solves.
 
 Each outer step divides the residual by about ``inner_rtol``, so rtol 1e-8
 takes two or three steps.  When a step fails to reduce the true residual by
@@ -51,7 +53,8 @@ class MixedBiCGStab:
     """complex64 BiCGStab corrections refined against the FP64 operator."""
 
     def __init__(self, config: SolverConfig, *, inner_rtol: float = 1e-5, max_outer: int = 12,
-                 stagnation: float = 0.5, inner_max_iter: int | None = None, preconditioner=None):
+                 stagnation: float = 0.5, inner_max_iter: int | None = None, preconditioner=None,
+                 preconditioned_max_iter: int = 8):
         if not (0.0 < inner_rtol < 1.0):
             raise ValueError(f"inner_rtol must lie in (0, 1), got {inner_rtol}")
         if max_outer < 1:
@@ -61,8 +64,10 @@ class MixedBiCGStab:
         self.max_outer = int(max_outer)
         self.stagnation = float(stagnation)
         self.inner_max_iter = inner_max_iter
+        if preconditioned_max_iter < 1:
+            raise ValueError(f"preconditioned_max_iter must be >= 1, got {preconditioned_max_iter}")
         self.preconditioner = preconditioner   # FNO: inner solves become right-preconditioned
-        self.preconditioned_max_iter = 100      # then a plain inner solve takes over
+        self.preconditioned_max_iter = int(preconditioned_max_iter)   # then plain inner solves take over
 
     def solve(self, operator, rhs) -> SolverResult:
         cfg = self.cfg
@@ -100,21 +105,24 @@ class MixedBiCGStab:
             want = max(self.inner_rtol, 0.5 * tol / rel)
             icfg = SolverConfig(max_iter=budget if pc is None else min(budget, self.preconditioned_max_iter),
                                 rtol=min(want, 0.5), chunk=cfg.chunk)
+            saved = (x.clone(), r.clone()) if pc is not None else None   # to undo an unhelpful partial step
             ires = comp.device_bicgstab(r32, icfg, precond=pc)
-            if pc is not None and ires.status != "converged":
-                # the learned preconditioner did not reach the inner tolerance: plain inner solve
-                inner_it += ires.iterations
-                mv32 += ires.matvec_count
-                inner_status.append(f"preconditioned {ires.status}")
-                ires = comp.device_bicgstab(r32, SolverConfig(max_iter=budget, rtol=icfg.rtol, chunk=cfg.chunk))
             inner_it += ires.iterations
             mv32 += ires.matvec_count
-            inner_status.append(ires.status)
+            pc_failed = pc is not None and ires.status != "converged"
+            inner_status.append(f"preconditioned {ires.status}" if pc is not None else ires.status)
+            if pc_failed:
+                pc = None   # the learned preconditioner missed the inner tolerance: plain inner solves from here
             convert_axpy(ires.u, rn, x, True)
             y = operator.matvec(x)
             mv64 += 1
             lincomb(1.0, f, -1.0, y, out=r)
             new = norm(r) / fn
+            if pc_failed and not (math.isfinite(new) and new < rel):
+                # the partial preconditioned correction did not help: undo it
+                x.copy_(saved[0])
+                r.copy_(saved[1])
+                new = rel
             hist.append(new)
             acts.append(mv32 + mv64)
             if not math.isfinite(new):
@@ -124,7 +132,7 @@ class MixedBiCGStab:
                 status = "converged"
                 rel = new
                 break
-            stalled = new > self.stagnation * rel
+            stalled = new > self.stagnation * rel and not pc_failed
             rel = new
             if stalled:
                 status = "stagnated"
