@@ -198,3 +198,38 @@ def test_training_reduces_loss_and_solves_stay_correct(em):
     assert mixed.status == "converged"
     true64 = float(torch.linalg.vector_norm(op64.matvec(mixed.u) - f64) / torch.linalg.vector_norm(f64))
     assert true64 < 1e-8
+
+
+def test_mixed_refinement_drops_a_harmful_preconditioner(em):
+    """A network with a large random output layer is a harmful M: the
+    preconditioned inner solve misses its tolerance within its budget, the
+    refinement keeps its partial correction only if the FP64 residual drops
+    (else restores x and r exactly), then finishes with plain inner solves
+    and meets the FP64 tolerance."""
+    import torch
+    from em3d_amd import learn
+    from em3d_amd.precond import FNO, FNOConfig
+    _say("[fno] harmful preconditioner in the mixed refinement, 32^3")
+    fno = FNO(FNOConfig(grid=16, channels=16, modes=8, layers=2), 3, 3, material_channels=7,
+              device=torch.device("cuda", 0), seed=5)
+    p = fno.params.cpu().numpy()
+    P = _split(fno, p)
+    P["w_out"][:] = np.random.default_rng(9).standard_normal(P["w_out"].size) * 30.0
+    fno.params.copy_(torch.as_tensor(p))
+    fno.refresh_bf16()
+    be64 = em.Backend.hip(em.Precision.DOUBLE)
+    op64, f64 = learn.material_operator(learn.random_material(2024), 32, be64)
+    cfg = em.SolverConfig(max_iter=300, rtol=1e-8)
+    plain = em.MixedBiCGStab(cfg).solve(op64, f64)
+    res = em.MixedBiCGStab(cfg, preconditioner=fno, preconditioned_max_iter=4).solve(op64, f64)
+    assert res.status == "converged", res.extras
+    assert res.extras["inner_status"][0].startswith("preconditioned")
+    assert not any(s.startswith("preconditioned") for s in res.extras["inner_status"][1:])
+    true = float(torch.linalg.vector_norm(op64.matvec(res.u) - f64) / torch.linalg.vector_norm(f64))
+    assert true < 1e-8
+    rel = float(torch.linalg.vector_norm(res.u - plain.u) / torch.linalg.vector_norm(plain.u))
+    assert rel < 1e-6
+    # the refinement's history never rises: an unhelpful partial step is undone
+    assert all(b <= a * (1 + 1e-12) for a, b in zip(res.residual_history, res.residual_history[1:]))
+    with pytest.raises(ValueError):
+        em.MixedBiCGStab(cfg, preconditioner=fno, preconditioned_max_iter=0)
