@@ -1,17 +1,19 @@
-// Fast pass bodies for power-of-two doubled axes (16 ... 512).
+// Fast pass bodies for doubled axes of 2^k (16 ... 512), 3 * 2^k and 5 * 2^k.
 //
-// Each transform of length L = R1 * R2 is a two-stage Stockham FFT held in
-// registers: stage A (radix R1) reads straight from global memory, with the
-// zero half of the doubled axis never loaded; one LDS transpose; stage B
-// (radix R2) writes straight to global memory, with the discarded half never
-// stored.  Lanes run along the contiguous dimension of each access (the
-// column index of a strided pass, the element index of a z pass), so every
-// global access of a wavefront is a run of 128-byte segments.
+// The z and strided passes run each transform of length L = R1 * R2 as a
+// two-stage Stockham FFT held in registers: stage A (radix R1) reads straight
+// from global memory, with the zero half of the doubled axis never loaded;
+// one LDS transpose; stage B (radix R2) writes straight to global memory,
+// with the discarded half never stored.  Lanes run along the contiguous
+// dimension of each access (the column index of a strided pass, the element
+// index of a z pass), so every global access of a wavefront is a run of
+// 128-byte segments.
 //
-// The x pass keeps the forward spectrum of all components in registers,
-// applies the 3x3 (or scalar) octant spectrum there and runs the inverse
-// transform from the same registers: the forward stage B and the inverse
-// stage A share one thread-to-butterfly mapping.
+// The x pass uses up to three register stages (radices RA, RB, RC) with an
+// LDS exchange between stages, keeps the forward spectrum of all components
+// in registers, applies the 3x3 (or scalar) octant spectrum there and runs the
+// inverse transform from the same registers: the forward last stage and the
+// inverse first stage share one thread-to-butterfly mapping.
 //
 // As in e3_blocks.hpp, kernels are phase functions of (thread id, per-thread
 // state) separated by block barriers, so the CPU tests can replay them.
