@@ -378,10 +378,15 @@ struct XR {
 #endif
 
 template <typename T, int L, int C, int TC_ = 128 / (int)sizeof(cpx<T>), bool STAGE_ = true, int WAVES_ = 2,
-          class RX = XRadix<L>>
+          class RX = XRadix<L>, bool PAIR_ = false>
 struct FastX {
   using Real = T;
   static constexpr int NCOMP = C;
+  // PAIR (scalar operator): the rows ky = p and Ly - p of a block run together
+  // as the two "components" c = 0, 1 of one schedule; they share every octant
+  // entry (the scalar kernel is even in ky), loaded once for both.
+  static constexpr bool PAIR = PAIR_ && C == 1;
+  static constexpr int CE = PAIR ? 2 : C;              // components of the phase schedule
   static constexpr int RA = RX::A, RB = RX::B, RC = RX::C;
   static constexpr int NTC = L / RC;                   // threads per column
   static constexpr int S = 64 / (int)sizeof(cpx<T>);   // octant group: 64 bytes of iz
@@ -407,7 +412,7 @@ struct FastX {
   static_assert(RA * (RB > 1 ? RB : 1) * RC == L, "radix plan");
   static_assert(RC % RA == 0 && (RB == 1 || RC % RB == 0), "stage radices must divide the last one");
   struct St {
-    cpx<T> X[C][RC];            // spectrum at kx = j + m (L / RC)
+    cpx<T> X[CE][RC];           // spectrum at kx = j + m (L / RC)
     cpx<T> mid[BB > 0 ? BB : 1][RB > 1 ? RB : 1];   // middle stage values in flight
     cpx<T> pre[BA][RA / 2];     // inputs of the next stage A, loaded one phase ahead
   };
@@ -478,10 +483,11 @@ struct FastX {
   // ---- forward stage A (Ns = 1): global rows < L/2 -> regs -> LDS ----
   // stage-A butterfly jj = q.j + bb NTC reads rows jj + m L/RA, live for m < RA/2
   E3_HD void fwd_a(int c, int tid, int h, St& s, int buf) const {
-    Lane q = lane(tid, h);
+    Lane q = lane(tid, prow(c, h));
     E3_OPAQUE(q.j);   // row addresses are formed per phase, not held across the row
     cpx<T>* t = seg(buf);
-    const cpx<T>* d = plane(c);
+    const cpx<T>* d = plane(pcomp(c));
+    const bool lv = live(c);
 #pragma unroll
     for (int bb = 0; bb < BA; ++bb) {
       const int jj = q.j + bb * NTC;
@@ -490,7 +496,7 @@ struct FastX {
       cpx<T> v[RA];
 #pragma unroll
       for (int m = 0; m < RA; ++m)
-        v[m] = m < RA / 2 ? *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) * sg : czero<T>();
+        v[m] = (m < RA / 2 && lv) ? *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) * sg : czero<T>();
       fft_reg<T, RA, false>(v);
 #pragma unroll
       for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
@@ -499,19 +505,20 @@ struct FastX {
   // stage A in two halves: the row loads of component c of row h into s.pre,
   // and (a phase later) the sign, the radix-RA butterflies and the LDS write
   E3_HD void load_a(int c, int tid, int h, St& s) const {
-    Lane q = lane(tid, h);
+    Lane q = lane(tid, prow(c, h));
     E3_OPAQUE(q.j);
-    const cpx<T>* d = plane(c);
+    const cpx<T>* d = plane(pcomp(c));
+    const bool lv = live(c);
 #pragma unroll
     for (int bb = 0; bb < BA; ++bb) {
       const int jj = q.j + bb * NTC;
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
 #pragma unroll
-      for (int m = 0; m < RA / 2; ++m) s.pre[bb][m] = *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b);
+      for (int m = 0; m < RA / 2; ++m) s.pre[bb][m] = lv ? *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) : czero<T>();
     }
   }
   E3_HD void fwd_a_pre(int c, int tid, int h, St& s, int buf) const {
-    const Lane q = lane(tid, h);
+    const Lane q = lane(tid, prow(c, h));
     cpx<T>* t = seg(buf);
     const T sg = comp_sign(c, q);
 #pragma unroll
@@ -569,8 +576,13 @@ struct FastX {
   // one row to the next in registers.
   static constexpr int MID = RB > 1 ? 1 : 0;
   static constexpr int PPC = 2 + 2 * MID;              // phases per component and direction (one tile)
-  static constexpr int NPH = PP ? (MID ? 4 * C + 1 : 2 * C + 1) : 2 * C * PPC;
-  E3_HD int rows() const { return two ? 2 : 1; }
+  static constexpr int NPH = PP ? (MID ? 4 * CE + 1 : 2 * CE + 1) : 2 * CE * PPC;
+  E3_HD int rows() const { return PAIR ? 1 : (two ? 2 : 1); }
+  // field component and row of schedule component c in row iteration h, and
+  // whether that row exists (a pair 0 or Ny has one row)
+  E3_HD int pcomp(int c) const { return PAIR ? 0 : c; }
+  E3_HD int prow(int c, int h) const { return PAIR ? c : h; }
+  E3_HD bool live(int c) const { return !PAIR || c == 0 || two; }
   int h_end = 1;       // rows() once the kernel starts the row loop (prefetch of the next row)
   template <int I>
   E3_HD void phase(int tid, int h, St& s) const {
@@ -597,54 +609,54 @@ struct FastX {
   E3_HD void phase_pp(int tid, int h, St& s) const {
     if constexpr (I == 0) {
       stage_a(0, tid, h, s, 0);
-      if constexpr (PREFETCH && !MID && C > 1) load_a(1, tid, h, s);
+      if constexpr (PREFETCH && !MID && CE > 1) load_a(1, tid, h, s);
     } else if constexpr (MID) {
-      constexpr bool fwd = I <= 2 * C;
-      constexpr int J = fwd ? I - 1 : I - 2 * C - 1;
+      constexpr bool fwd = I <= 2 * CE;
+      constexpr int J = fwd ? I - 1 : I - 2 * CE - 1;
       constexpr int c = J / 2;
       if constexpr (J % 2 == 0) {
-        if constexpr (PREFETCH && fwd && c + 1 < C) load_a(c + 1, tid, h, s);
+        if constexpr (PREFETCH && fwd && c + 1 < CE) load_a(c + 1, tid, h, s);
         mid_read(tid, h, s, 0);
         mid_write<!fwd>(tid, h, s, fwd ? RA : RC, 1);
       } else if constexpr (fwd) {
         fwd_c(c, tid, h, s, 1);
-        if constexpr (c + 1 < C) {
+        if constexpr (c + 1 < CE) {
           stage_a(c + 1, tid, h, s, 0);
         } else {
           multiply(tid, h, s);
           inv_c(0, tid, h, s, 0);
         }
       } else {
-        if constexpr (PREFETCH && c + 1 == C) {
+        if constexpr (PREFETCH && c + 1 == CE) {
           if (h + 1 < h_end) load_a(0, tid, h + 1, s);
         }
         inv_a(c, tid, h, 1);
-        if constexpr (c + 1 < C) inv_c(c + 1, tid, h, s, 0);
+        if constexpr (c + 1 < CE) inv_c(c + 1, tid, h, s, 0);
       }
-    } else if constexpr (I <= C) {
+    } else if constexpr (I <= CE) {
       constexpr int c = I - 1;
       fwd_c(c, tid, h, s, c % 2);
-      if constexpr (c + 1 < C) {
+      if constexpr (c + 1 < CE) {
         stage_a(c + 1, tid, h, s, (c + 1) % 2);             // consumes s.pre ...
-        if constexpr (PREFETCH && c + 2 < C) load_a(c + 2, tid, h, s);   // ... before it is refilled
+        if constexpr (PREFETCH && c + 2 < CE) load_a(c + 2, tid, h, s);   // ... before it is refilled
       } else {
         multiply(tid, h, s);
-        inv_c(0, tid, h, s, C % 2);
+        inv_c(0, tid, h, s, CE % 2);
       }
     } else {
-      constexpr int c = I - C - 1;
-      if constexpr (PREFETCH && c + 1 == C) {
+      constexpr int c = I - CE - 1;
+      if constexpr (PREFETCH && c + 1 == CE) {
         if (h + 1 < h_end) load_a(0, tid, h + 1, s);
       }
-      inv_a(c, tid, h, (C + c) % 2);
-      if constexpr (c + 1 < C) inv_c(c + 1, tid, h, s, (C + c + 1) % 2);
+      inv_a(c, tid, h, (CE + c) % 2);
+      if constexpr (c + 1 < CE) inv_c(c + 1, tid, h, s, (CE + c + 1) % 2);
     }
   }
   // One tile, stages in place (the middle stage reads, then writes after a barrier).
   template <int I>
   E3_HD void phase_one(int tid, int h, St& s) const {
-    constexpr bool fwd = I < C * PPC;
-    constexpr int ii = fwd ? I : I - C * PPC;
+    constexpr bool fwd = I < CE * PPC;
+    constexpr int ii = fwd ? I : I - CE * PPC;
     constexpr int c = ii / PPC, r = ii - c * PPC;
     if constexpr (fwd) {
       if constexpr (r == 0) {
@@ -725,8 +737,10 @@ struct FastX {
       const bool fx = (RC % 2 == 0) ? (m >= RC / 2) : (q.j + m * NTC > L / 2);
       const int off = koff(q, t0, m);
       if constexpr (C == 1) {
-        const cpx<T> k = kload(base, off);
-        s.X[0][m] = cmul_f<false>(mk(k.re, k.im * sj), s.X[0][m]);
+        const cpx<T> k0 = kload(base, off);
+        const cpx<T> k = mk(k0.re, k0.im * sj);
+        s.X[0][m] = cmul_f<false>(k, s.X[0][m]);
+        if constexpr (PAIR) s.X[1][m] = cmul_f<false>(k, s.X[1][m]);
       } else {
         cpx<T> k[6];
 #pragma unroll
@@ -765,11 +779,12 @@ struct FastX {
   }
   // ---- inverse last stage (radix RA, Ns = L/RA): LDS -> rows < Nx ----
   E3_HD void inv_a(int c, int tid, int h, int buf) const {
-    Lane q = lane(tid, h);
+    Lane q = lane(tid, prow(c, h));
     E3_OPAQUE(q.j);
     const cpx<T>* t = seg(buf);
     const cpx<T>* tw = twl();
-    cpx<T>* d = plane(c);
+    cpx<T>* d = plane(pcomp(c));
+    if (!live(c)) return;   // block-uniform: the row does not exist
 #pragma unroll
     for (int bb = 0; bb < BA; ++bb) {
       const int jj = q.j + bb * NTC;
@@ -1088,10 +1103,13 @@ bool fast_x3(Run& run, const XArgs& a) {
 #ifndef E3_X1_STAGE
 #define E3_X1_STAGE true
 #endif
+#ifndef E3_X1_PAIR
+#define E3_X1_PAIR 1   // both rows of a ky pair in one schedule
+#endif
 
 template <typename T, int L, class Run>
 bool fast_x_len(Run& run, const XArgs& a) {
-  using X1 = FastX<T, L, 1, E3_X1_TC, E3_X1_STAGE>;
+  using X1 = FastX<T, L, 1, E3_X1_TC, E3_X1_STAGE, 2, XRadix<L>, E3_X1_PAIR>;
   if (a.N[0] * 2 != L || !a.perm) return false;
   if (a.C == 3) {
     constexpr int TCd = E3_X3_TC;
