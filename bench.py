@@ -315,6 +315,62 @@ def profile_case(args):
     print(json.dumps({"case": args.case, "ms_per_matvec": round(t / args.steps * 1e3, 4)}), flush=True)
 
 
+def learned_eval(args):
+    """The learned-preconditioner protocol: train the FNO on seeded random
+    materials (seeds 0 .. n-1, solved at --train-grid^3), then on held-out
+    materials (seeds 100000 + i) at N^3 time plain complex128 BiCGStab, the
+    complex64/FP64 mixed refinement and the learned-PC mixed refinement to
+    true rtol 1e-8; one JSON line with the distributions."""
+    import torch
+
+    import em3d_amd as em
+    from em3d_amd import learn
+    fno, info = train_preconditioner(em, args.train_materials, args.train_steps, args.train_grid)
+    be64 = em.Backend.hip(em.Precision.DOUBLE)
+    cfg = em.SolverConfig(max_iter=1000, rtol=1e-8)
+    rows = []
+    for i in range(args.eval_materials):
+        mat = learn.random_material(100000 + i)
+        op, f = learn.material_operator(mat, args.n, be64)
+        op.single_precision_companion().matvec(f.to(torch.complex64))   # one-time complex64 set-up
+        plain, t_plain = _timed(lambda: em.BiCGStab(cfg).solve(op, f))
+        mixed, t_mixed = _timed(lambda: em.MixedBiCGStab(cfg).solve(op, f))
+        lm, t_lm = _timed(lambda: em.MixedBiCGStab(cfg, preconditioner=fno).solve(op, f))
+        rows.append({"seed": 100000 + i, "k0": round(mat.k0, 3), "inclusions": len(mat.inclusions),
+                     "fp64_s": t_plain, "fp64_it": plain.iterations, "fp64_ok": plain.status == "converged",
+                     "mixed_s": t_mixed, "mixed_it": mixed.iterations,
+                     "mixed_true": _true_residual(op, mixed.u, f),
+                     "learned_s": t_lm, "learned_it": lm.iterations, "learned_true": _true_residual(op, lm.u, f),
+                     "learned_pc_converged": bool(lm.extras["inner_status"])
+                     and lm.extras["inner_status"][0] == "preconditioned converged"})
+        if (i + 1) % 20 == 0:
+            log(f"[bench] learned-eval: {i + 1} materials")
+        del op, f, plain, mixed, lm
+        torch.cuda.empty_cache()
+
+    def dist(key):
+        v = np.array([r[key] for r in rows], dtype=float)
+        return {"median": round(float(np.median(v)), 5), "mean": round(float(v.mean()), 5),
+                "min": round(float(v.min()), 5), "max": round(float(v.max()), 5)}
+    sp_mixed = [r["fp64_s"] / r["mixed_s"] for r in rows]
+    sp_learned = [r["fp64_s"] / r["learned_s"] for r in rows]
+    out = {"case": "learned-eval", "grid": args.n, "training": info,
+           "held_out_materials": len(rows),
+           "fp64_bicgstab_s": dist("fp64_s"), "fp64_iterations": dist("fp64_it"),
+           "mixed_s": dist("mixed_s"), "learned_pc_mixed_s": dist("learned_s"),
+           "learned_pc_mixed_iterations": dist("learned_it"),
+           "speedup_mixed_vs_fp64": {"median": round(float(np.median(sp_mixed)), 3),
+                                     "min": round(float(np.min(sp_mixed)), 3), "max": round(float(np.max(sp_mixed)), 3)},
+           "speedup_learned_pc_mixed_vs_fp64": {"median": round(float(np.median(sp_learned)), 3),
+                                                "min": round(float(np.min(sp_learned)), 3),
+                                                "max": round(float(np.max(sp_learned)), 3)},
+           "preconditioned_inner_converged": int(sum(r["learned_pc_converged"] for r in rows)),
+           "all_fp64_converged": all(r["fp64_ok"] for r in rows),
+           "max_true_residual_mixed": max(r["mixed_true"] for r in rows),
+           "max_true_residual_learned": max(r["learned_true"] for r in rows)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,8 +380,13 @@ def main():
     ap.add_argument("--k0", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--case", default="full", choices=["full", "em-fp64", "acoustic-256"],
-                    help="full: the benchmark; em-fp64 / acoustic-256: only that matvec, for profiling")
+    ap.add_argument("--case", default="full", choices=["full", "em-fp64", "acoustic-256", "learned-eval"],
+                    help="full: the benchmark; em-fp64 / acoustic-256: only that matvec, for profiling; "
+                         "learned-eval: train the FNO and compare solvers on held-out materials")
+    ap.add_argument("--train-materials", type=int, default=500)
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--train-grid", type=int, default=64)
+    ap.add_argument("--eval-materials", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-acoustic", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="time the default kernel variants")
@@ -338,6 +399,8 @@ def main():
                     help="also run 3 plain device copies of 512 MiB (read 512 MiB, write 512 MiB each), "
                          "the calibration kernel for HBM traffic counters")
     args = ap.parse_args()
+    if args.case == "learned-eval":
+        return learned_eval(args)
     if args.case != "full":
         return profile_case(args)
     if args.gpus != 1:
