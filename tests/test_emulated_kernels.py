@@ -142,10 +142,13 @@ def test_fast_kernels_match_oracle(lib, N, L, k, prec, tol):
         assert _rel(y[b], vie.em_matvec(chi, u[b], Kh)) < tol
         assert _rel(ya[b], vie.em_rmatvec(chi, u[b], Khat=Kh)) < tol
     ca = _rand(N, 10) * 0.3
-    ua = _rand(N, 11)
-    opa = emu.EmuOperator(lib, N, L, k, ca, 1, precision=prec)
-    assert _rel(opa.matvec(ua), vie.acoustic_matvec(ca, ua, vie.acoustic_kernel_hat(N, L, k))) < tol
-    assert _rel(opa.rmatvec(ua), vie.acoustic_rmatvec(ca, ua, vie.acoustic_kernel_hat(N, L, k, adjoint=True))) < tol
+    ua = _rand((2,) + N, 11)
+    opa = emu.EmuOperator(lib, N, L, k, ca, 1, precision=prec, batch=2)
+    Ka, Kaa = vie.acoustic_kernel_hat(N, L, k), vie.acoustic_kernel_hat(N, L, k, adjoint=True)
+    ya, yaa = opa.matvec(ua), opa.rmatvec(ua)
+    for b in range(2):
+        assert _rel(ya[b], vie.acoustic_matvec(ca, ua[b], Ka)) < tol
+        assert _rel(yaa[b], vie.acoustic_rmatvec(ca, ua[b], Kaa)) < tol
     used = [lib.emu_fast_calls(i) - before[i] for i in range(4)]
     fast_len = {16, 32, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512}
     if 2 * N[2] in fast_len:
