@@ -383,8 +383,8 @@ def main():
     ap.add_argument("--case", default="full", choices=["full", "em-fp64", "acoustic-256", "learned-eval"],
                     help="full: the benchmark; em-fp64 / acoustic-256: only that matvec, for profiling; "
                          "learned-eval: train the FNO and compare solvers on held-out materials")
-    ap.add_argument("--train-materials", type=int, default=500)
-    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--train-materials", type=int, default=2000)
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-grid", type=int, default=64)
     ap.add_argument("--eval-materials", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")

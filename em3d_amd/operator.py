@@ -91,6 +91,10 @@ class HipPlan:
             self.handle = None
 
 
+# FFT-pass scratch shared by the operators of one plan on one stream
+_SHARED_WORK: dict = {}
+
+
 def khat_storage_index(N, ncomp: int, elem_bytes: int):
     """Storage index of every logical octant entry (comp, ix, iy, iz) as an int64
     array of shape (K, Nx+1, Ny+1, Nz+1).  The library stores the octant as
@@ -291,12 +295,24 @@ class _FieldApplier:
     def _field_shape(self):
         return ((3,) if self._ncomp == 3 else ()) + self._N
 
-    def _workspace(self, plan):
+    def _workspace(self, plan, private: bool = False):
+        """Scratch of the FFT passes for `plan`.  Operators of one shape share
+        one workspace per stream (their matvecs run one after another in
+        stream order), so many same-shape materials cost one workspace;
+        ``private`` gives this operator its own (packed groups, whose members
+        run as concurrent graph branches)."""
         import torch
-        w = self._work.get(plan.batch)
+        if private:
+            w = self._work.get(plan.batch)
+            if w is None:
+                w = torch.empty(plan.work_bytes, dtype=torch.uint8, device=self._be.torch_device)
+                self._work[plan.batch] = w
+            return w
+        key = (id(plan), self._be.device_index, _lib.stream_handle(self._be.torch_device))   # plans are cached for the process
+        w = _SHARED_WORK.get(key)
         if w is None:
             w = torch.empty(plan.work_bytes, dtype=torch.uint8, device=self._be.torch_device)
-            self._work[plan.batch] = w
+            _SHARED_WORK[key] = w
         return w
 
     def _prepare_input(self, field):

@@ -34,7 +34,7 @@ class PackedOperators:
             khat, kind, chi = op._spectrum()
             u = torch.zeros(op._field_shape(), dtype=be.precision.torch_complex_dtype, device=be.torch_device)
             y = torch.zeros_like(u)
-            w = op._workspace(plan)
+            w = op._workspace(plan, private=True)   # members run as concurrent graph branches
             self.inputs.append(u)
             self.outputs.append(y)
             keep += [plan, khat, chi, w]
