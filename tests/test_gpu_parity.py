@@ -455,3 +455,23 @@ def test_reference_format_records(em):
     res, run = reports.timed_solve(em.BiCGStab(em.SolverConfig(rtol=1e-8)), op, prob.wave, case_name="ellipsoid_N8",
                                    solver_name="BiCGStab")
     assert run.iterations == res.iterations and run.final_residual == res.residual_history[-1]
+
+
+def test_same_shape_operators_share_a_workspace(em):
+    """Operators of one shape share one FFT workspace per stream; their
+    interleaved matvecs, and an adjoint between them, equal each operator's
+    own results."""
+    import torch
+    _say("[workspace] two same-shape operators, interleaved")
+    be = _be(em, "single")
+    N, L = (24, 20, 16), (1.0, 0.8, 0.7)
+    ops = [em.Operator(_problem(em, be, N, L, k, _chi(kind, N, seed=s)))
+           for k, kind, s in ((3.0, "diag", 11), (5.0, "full", 12))]
+    u = be.array(_rand((3,) + N, 7).astype(be.complex_dtype))
+    ref = [op.matvec(u).clone() for op in ops]
+    refa = ops[0].rmatvec(u).clone()
+    for _ in range(2):
+        y1 = ops[1].matvec(u)
+        ya = ops[0].rmatvec(u)
+        y0 = ops[0].matvec(u)
+        assert torch.equal(y0, ref[0]) and torch.equal(y1, ref[1]) and torch.equal(ya, refa)
