@@ -54,7 +54,7 @@ def _integral(u, grid, k0: float, chi_kind: int, chi, ncomp: int, directions):
     M = dirs.shape[0]
     out = torch.empty((batch, M, ncomp), dtype=torch.complex128, device=dev)
     lib = _lib.lib()
-    work = torch.empty(int(lib.em3d_scatter_work_bytes(batch, M, ncomp)), dtype=torch.uint8, device=dev)
+    work = torch.empty(int(lib.em3d_scatter_work_bytes(*grid.N, batch, M, ncomp)), dtype=torch.uint8, device=dev)
     _lib.check(lib.em3d_scatter_integral(*grid.N, grid.x.data_ptr(), grid.y.data_ptr(), grid.z.data_ptr(),
                                          be.precision.code, batch, ncomp, chi.data_ptr(), chi_kind, t.data_ptr(), M,
                                          d.data_ptr(), float(k0), float(grid.dv), out.data_ptr(), work.data_ptr(),

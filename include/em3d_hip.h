@@ -181,8 +181,9 @@ int em3d_convert_axpy(long long n, int src_precision, const void* src, double a_
  * acoustics/farfield.py for ncomp = 1), accumulated in FP64.  u: (batch,
  * ncomp, N) fields, chi packed as for em3d_matvec (acoustic: chi = eta - 1,
  * kind 1); dirs: DEVICE (ndir, 3) doubles; out: DEVICE (batch, ndir, ncomp)
- * complex128; work: em3d_scatter_work_bytes(batch, ndir, ncomp). */
-size_t em3d_scatter_work_bytes(int batch, int ndir, int ncomp);
+ * complex128; work: em3d_scatter_work_bytes(nx, ny, nz, batch, ndir, ncomp)
+ * bytes (partial sums and the per-axis phase tables). */
+size_t em3d_scatter_work_bytes(int nx, int ny, int nz, int batch, int ndir, int ncomp);
 int em3d_scatter_integral(int nx, int ny, int nz, const void* ax, const void* ay, const void* az, int precision,
                           int batch, int ncomp, const void* chi, int chi_kind, const void* u, int ndir,
                           const double* dirs, double k0, double dv, double* out, void* work, hipStream_t stream);

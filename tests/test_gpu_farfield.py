@@ -71,6 +71,38 @@ def test_scatter_integral_matches_oracle(em, prec, tol):
         assert _rel(got, ref) < tol
 
 
+@pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 2e-6)])
+def test_scatter_integral_ragged_tiles(em, prec, tol):
+    """Sizes that leave every tile of the kernel partly filled: 70 y rows (64-row tiles), 37 z
+    (16-z chunks), 70 directions (64-direction tiles), a full 3 x 3 contrast and two fields; the
+    scalar case on 65 rows and 20 z."""
+    from oracle import farfield as ff
+    _say(f"[farfield] ragged tiles {prec}")
+    be = em.Backend.hip(em.Precision.DOUBLE if prec == "double" else em.Precision.SINGLE)
+    rng = np.random.default_rng(11)
+    N, L, c, k0 = (3, 70, 37), (0.3, 1.4, 0.9), (0.1, 0.0, -0.05), 9.0
+    chi = ((rng.standard_normal((3, 3) + N) + 1j * rng.standard_normal((3, 3) + N)) * 0.3).astype(be.complex_dtype)
+    u = (rng.standard_normal((2, 3) + N) + 1j * rng.standard_normal((2, 3) + N)).astype(be.complex_dtype)
+    grid = em.Grid(N=N, L=L, center=c, backend=be)
+    prob = em.Problem(grid=grid, eps_tensor=be.array(chi), wave=be.array(u[0]), k0=k0, volume=1.0)
+    dirs = rng.standard_normal((70, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    F = em.farfield.scatter_integral(be.array(u), prob, dirs)
+    for b in range(2):
+        ref = ff.scatter_integral(u[b].astype(np.complex128), chi.astype(np.complex128), N, L, c, k0, dirs)
+        assert _rel(F[b], ref) < tol
+    from em3d_amd.acoustics import farfield as af
+    N2, k2 = (2, 65, 20), 6.0
+    grid2 = em.Grid(N=N2, L=(0.2, 1.0, 0.5), center=(0, 0, 0), backend=be)
+    eta = (1.0 + 0.5 * (rng.standard_normal(N2) + 1j * rng.standard_normal(N2))).astype(be.complex_dtype)
+    u2 = (rng.standard_normal(N2) + 1j * rng.standard_normal(N2)).astype(be.complex_dtype)
+    p2 = em.acoustics.AcousticProblem(grid=grid2, eta=be.array(eta), wave=be.array(u2), k0=k2, volume=1.0)
+    amp = af.farfield_amplitude(be.array(u2), p2, dirs)
+    ref = k2 ** 2 / (4 * np.pi) * ff.scatter_integral(u2.astype(np.complex128), eta.astype(np.complex128) - 1.0,
+                                                      N2, (0.2, 1.0, 0.5), (0, 0, 0), k2, dirs)
+    assert _rel(amp, ref) < tol
+
+
 def test_acoustic_farfield_matches_oracle(em):
     from oracle import farfield as ff
     from em3d_amd.acoustics import farfield as af
