@@ -70,17 +70,30 @@ __device__ __forceinline__ cpx<double> pol_at(const cpx<T>* chi, int chi_kind, c
 constexpr int kFR = 4, kFD = 4;         // rows (y) x directions per thread
 constexpr int kRT = 16 * kFR;           // y rows per tile
 constexpr int kDT = 16 * kFD;           // directions per tile
-constexpr int kKZ = 16;                 // z per LDS chunk
-constexpr int kKP = kKZ + 1;            // padded LDS row
+#ifndef E3_FF_KZ
+#define E3_FF_KZ 16                     // z per LDS chunk
+#endif
 #ifndef E3_FF_UNROLL
 #define E3_FF_UNROLL 1                  // unroll of the z loop of a chunk
 #endif
+#ifndef E3_FF_PREFETCH
+#define E3_FF_PREFETCH 0                // 1: next chunk's loads in flight during the products (more VGPRs)
+#endif
+#ifndef E3_FF_WAVES
+#define E3_FF_WAVES 1                   // minimum waves per SIMD the register budget must allow
+#endif
+constexpr int kKZ = E3_FF_KZ;
+constexpr int kKP = kKZ + 1;            // padded LDS row
 
-// grid (nx, direction tiles, batch); part[b][m][a][x] (complex)
+// grid (nx, direction tiles, batch); part[b][m][a][x] (complex).  The block
+// walks chunks ch = (component a, y tile, z chunk) in that order; a chunk's
+// p and Z values are loaded into registers (one chunk ahead with
+// E3_FF_PREFETCH) and stored to LDS between the two barriers that open it.
 template <typename T, int C>
-__global__ __launch_bounds__(kT) void k_scatter(int nx, int ny, int nz, const cpx<T>* chi, int chi_kind,
+__global__ __launch_bounds__(kT, E3_FF_WAVES) void k_scatter(int nx, int ny, int nz, const cpx<T>* chi, int chi_kind,
                                                 const cpx<T>* u, int ndir, const cpx<double>* tab,
                                                 cpx<double>* part) {
+  constexpr int NQ = kRT * kKZ / kT, NQZ = kDT * kKZ / kT;   // values per thread per chunk
   __shared__ cpx<double> sp[kRT * kKP];   // p[y][z] of the chunk; the y-sum reduction afterwards
   __shared__ cpx<double> sz[kDT * kKP];   // Z[d][z] of the chunk
   const long long M = (long long)nx * ny * nz;
@@ -90,59 +103,78 @@ __global__ __launch_bounds__(kT) void k_scatter(int nx, int ny, int nz, const cp
   const int nd = ndir - m0 < kDT ? ndir - m0 : kDT;   // live directions of the tile
   const int tid = threadIdx.x, td = tid & 15, tr = tid >> 4;
   const long long xb = (long long)ix * ny * nz;
-  for (int a = 0; a < C; ++a) {
-    cpx<double> s[kFD];
+  const int nyt = (ny + kRT - 1) / kRT, nzt = (nz + kKZ - 1) / kKZ, nch = C * nyt * nzt;
+  cpx<double> pv[NQ], zv[NQZ];
+  auto fetch = [&](int ch) {
+    const int a = ch / (nyt * nzt), rem = ch - a * nyt * nzt;
+    const int y0 = (rem / nzt) * kRT, z0 = (rem % nzt) * kKZ;
 #pragma unroll
-    for (int k = 0; k < kFD; ++k) s[k] = mk(0.0, 0.0);
-    for (int y0 = 0; y0 < ny; y0 += kRT) {
-      cpx<double> acc[kFR][kFD];
+    for (int q = 0; q < NQ; ++q) {
+      const int e = tid + q * kT, y = y0 + e / kKZ, z = z0 + e % kKZ;
+      pv[q] = (y < ny && z < nz) ? pol_at<T, C>(chi, chi_kind, ub, M, xb + (long long)y * nz + z, a) : mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < NQZ; ++q) {
+      const int e = tid + q * kT, d = e / kKZ, z = z0 + e % kKZ;
+      zv[q] = (d < nd && z < nz) ? tab[(size_t)(m0 + d) * S + nx + ny + z] : mk(0.0, 0.0);
+    }
+  };
+  cpx<double> s[kFD], acc[kFR][kFD];
+#pragma unroll
+  for (int k = 0; k < kFD; ++k) {
+    s[k] = mk(0.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < kFR; ++j) acc[j][k] = mk(0.0, 0.0);
+  }
+  if (E3_FF_PREFETCH) fetch(0);
+  for (int ch = 0; ch < nch; ++ch) {
+    const int a = ch / (nyt * nzt), rem = ch - a * nyt * nzt, yt = rem / nzt, zt = rem - yt * nzt;
+    __syncthreads();
+    if (!E3_FF_PREFETCH) fetch(ch);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int e = tid + q * kT;
+      sp[(e / kKZ) * kKP + e % kKZ] = pv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NQZ; ++q) {
+      const int e = tid + q * kT;
+      sz[(e / kKZ) * kKP + e % kKZ] = zv[q];
+    }
+    __syncthreads();
+    if (E3_FF_PREFETCH && ch + 1 < nch) fetch(ch + 1);
+#pragma unroll E3_FF_UNROLL
+    for (int kz = 0; kz < kKZ; ++kz) {
+      cpx<double> p[kFR], w[kFD];
+#pragma unroll
+      for (int j = 0; j < kFR; ++j) p[j] = sp[(tr + 16 * j) * kKP + kz];
+#pragma unroll
+      for (int k = 0; k < kFD; ++k) w[k] = sz[(td + 16 * k) * kKP + kz];
 #pragma unroll
       for (int j = 0; j < kFR; ++j)
 #pragma unroll
-        for (int k = 0; k < kFD; ++k) acc[j][k] = mk(0.0, 0.0);
-      for (int z0 = 0; z0 < nz; z0 += kKZ) {
-        __syncthreads();
+        for (int k = 0; k < kFD; ++k) cmac_f<false>(acc[j][k], w[k], p[j]);
+    }
+    if (zt + 1 < nzt) continue;
+    // end of a y tile: s_d += Y_d(y) acc(y, d)
 #pragma unroll
-        for (int q = 0; q < kRT * kKZ / kT; ++q) {
-          const int e = tid + q * kT, kz = e % kKZ, r = e / kKZ;
-          const int y = y0 + r, z = z0 + kz;
-          sp[r * kKP + kz] = (y < ny && z < nz) ? pol_at<T, C>(chi, chi_kind, ub, M, xb + (long long)y * nz + z, a)
-                                                : mk(0.0, 0.0);
-        }
+    for (int j = 0; j < kFR; ++j) {
+      const int y = yt * kRT + tr + 16 * j;
 #pragma unroll
-        for (int q = 0; q < kDT * kKZ / kT; ++q) {
-          const int e = tid + q * kT, kz = e % kKZ, d = e / kKZ;
-          const int z = z0 + kz;
-          sz[d * kKP + kz] = (d < nd && z < nz) ? tab[(size_t)(m0 + d) * S + nx + ny + z] : mk(0.0, 0.0);
-        }
-        __syncthreads();
-#pragma unroll E3_FF_UNROLL
-        for (int kz = 0; kz < kKZ; ++kz) {
-          cpx<double> p[kFR], w[kFD];
-#pragma unroll
-          for (int j = 0; j < kFR; ++j) p[j] = sp[(tr + 16 * j) * kKP + kz];
-#pragma unroll
-          for (int k = 0; k < kFD; ++k) w[k] = sz[(td + 16 * k) * kKP + kz];
-#pragma unroll
-          for (int j = 0; j < kFR; ++j)
-#pragma unroll
-            for (int k = 0; k < kFD; ++k) cmac_f<false>(acc[j][k], w[k], p[j]);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < kFR; ++j) {
-        const int y = y0 + tr + 16 * j;
-        if (y >= ny) continue;
-#pragma unroll
-        for (int k = 0; k < kFD; ++k) {
-          const int d = td + 16 * k;
-          if (d < nd) cmac_f<false>(s[k], tab[(size_t)(m0 + d) * S + nx + y], acc[j][k]);
-        }
+      for (int k = 0; k < kFD; ++k) {
+        const int d = td + 16 * k;
+        if (y < ny && d < nd) cmac_f<false>(s[k], tab[(size_t)(m0 + d) * S + nx + y], acc[j][k]);
+        acc[j][k] = mk(0.0, 0.0);
       }
     }
+    if (yt + 1 < nyt) continue;
+    // end of component a: sum s over the 16 row groups, times X_d(x)
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kFD; ++k) sp[tr * kDT + td + 16 * k] = s[k];
+    for (int k = 0; k < kFD; ++k) {
+      sp[tr * kDT + td + 16 * k] = s[k];
+      s[k] = mk(0.0, 0.0);
+    }
     __syncthreads();
     if (tid < nd) {
       cpx<double> v = mk(0.0, 0.0);
