@@ -60,8 +60,8 @@ SIGNATURES = {
                            C.POINTER(SolverResultC), _VP]),
     "em3d_convert_axpy": (_I, [_LL, _I, _VP, _D, _D, _I, _VP, _I, _VP]),
     "em3d_scatter_work_bytes": (_SZ, [_I, _I, _I, _I, _I, _I]),
-    "em3d_scatter_integral": (_I, [_I, _I, _I, _VP, _VP, _VP, _I, _I, _I, _VP, _I, _VP, _I, _VP, _D, _D, _VP, _VP,
-                                   _VP]),
+    "em3d_scatter_integral": (_I, [_I, _I, _I, _VP, _VP, _VP, _I, _I, _I, _VP, _I, _VP, _I, _VP, _I, _D, _D, _VP,
+                                   _VP, _VP]),
     "em3d_plane_waves": (_I, [_I, _I, _I, _VP, _VP, _VP, _I, _I, _VP, _D, _I, _VP, _VP]),
     "em3d_ellipsoid_contrast": (_I, [_I, _I, _I, _VP, _VP, _VP, _I, _I, _VP, _VP, _D, _I, _VP, _VP]),
     "em3d_eta_inclusions": (_I, [_I, _I, _I, _VP, _VP, _VP, _I, _I, _VP, _D, _D, _VP, _VP]),

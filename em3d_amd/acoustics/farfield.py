@@ -5,7 +5,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from ..farfield import _integral
+from ..farfield import _integral, packed_contrast
 from ..operator import CHI_ISO
 
 
@@ -38,7 +38,8 @@ def farfield_amplitude(u, problem, directions, *, batch_size: int = 64) -> np.nd
     shape = getattr(u, "shape", None)
     if shape is not None and tuple(shape) != tuple(grid.N):
         raise ValueError(f"u.shape {tuple(shape)} != expected {tuple(grid.N)}")
-    F = _integral(u, grid, problem.k0, CHI_ISO, problem.chi.contiguous(), 1, dirs)[0, :, 0]
+    chi = packed_contrast(problem.eta, lambda eta: (eta - 1.0).contiguous())   # problem.chi, packed once
+    F = _integral(u, grid, problem.k0, CHI_ISO, chi, 1, dirs)[0, :, 0]
     coeff = float(problem.k0) ** 2 / (4.0 * np.pi)     # dv is applied by the integral
     return coeff * F.cpu().numpy()
 

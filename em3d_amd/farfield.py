@@ -7,10 +7,32 @@ FP64 accumulation) for one field or a batch of fields at once.
 """
 from __future__ import annotations
 
+import weakref
+
 import numpy as np
 
 from . import _lib
 from .operator import pack_contrast, require_hip
+
+_PACKED: dict = {}
+
+
+def packed_contrast(t, make):
+    """make(t), kept while t is alive and unchanged: the same tensor object
+    with the same version counter (torch bumps it on every in-place change).
+    The entry is dropped when t is freed.  Repeated far-field calls on one
+    problem (a sweep of planes, of fields) then pack the contrast once."""
+    if not hasattr(t, "_version"):
+        return make(t)
+    key = id(t)
+    hit = _PACKED.get(key)
+    if hit is not None and hit[0]() is t and hit[1] == t._version:
+        return hit[2]
+    val = make(t)
+    if hit is None:
+        weakref.finalize(t, _PACKED.pop, key, None)
+    _PACKED[key] = (weakref.ref(t), t._version, val)
+    return val
 
 __all__ = ["scatter_integral", "scatter_integral_device", "rcs", "rcs_plane", "plane_directions"]
 
@@ -28,6 +50,16 @@ def plane_directions(plane: str, n_phi: int):
     else:
         raise ValueError(f"plane must be 'xy'|'yz'|'xz', got {plane!r}")
     return phi, dirs
+
+
+def shared_axis(dirs) -> int:
+    """The axis (2, 0, 1 tried in that order) along which all directions have
+    exactly the same component, else -1.  Plane cuts have one: the kernel then
+    sums that axis once for all directions."""
+    for a in (2, 0, 1):
+        if np.all(dirs[:, a] == dirs[0, a]):
+            return a
+    return -1
 
 
 def _integral(u, grid, k0: float, chi_kind: int, chi, ncomp: int, directions):
@@ -57,15 +89,15 @@ def _integral(u, grid, k0: float, chi_kind: int, chi, ncomp: int, directions):
     work = torch.empty(int(lib.em3d_scatter_work_bytes(*grid.N, batch, M, ncomp)), dtype=torch.uint8, device=dev)
     _lib.check(lib.em3d_scatter_integral(*grid.N, grid.x.data_ptr(), grid.y.data_ptr(), grid.z.data_ptr(),
                                          be.precision.code, batch, ncomp, chi.data_ptr(), chi_kind, t.data_ptr(), M,
-                                         d.data_ptr(), float(k0), float(grid.dv), out.data_ptr(), work.data_ptr(),
-                                         _lib.stream_handle(dev)), "em3d_scatter_integral")
+                                         d.data_ptr(), shared_axis(dirs), float(k0), float(grid.dv), out.data_ptr(),
+                                         work.data_ptr(), _lib.stream_handle(dev)), "em3d_scatter_integral")
     return out
 
 
 def scatter_integral_device(u, problem, directions):
     """F[b][m] = dv sum_q (chi u_b)_q exp(-i k0 d_m . r_q) as a device tensor
     (batch, M, 3) complex128 (u: (3, N) or (batch, 3, N))."""
-    kind, chi = pack_contrast(problem.eps_tensor)
+    kind, chi = packed_contrast(problem.eps_tensor, pack_contrast)
     return _integral(u, problem.grid, problem.k0, kind, chi, 3, directions)
 
 

@@ -182,11 +182,17 @@ int em3d_convert_axpy(long long n, int src_precision, const void* src, double a_
  * ncomp, N) fields, chi packed as for em3d_matvec (acoustic: chi = eta - 1,
  * kind 1); dirs: DEVICE (ndir, 3) doubles; out: DEVICE (batch, ndir, ncomp)
  * complex128; work: em3d_scatter_work_bytes(nx, ny, nz, batch, ndir, ncomp)
- * bytes (partial sums and the per-axis phase tables). */
+ * bytes (partial sums, per-axis phase tables, a collapsed field).
+ * shared_axis: -1, or the axis (0 x, 1 y, 2 z) along which every direction
+ * has the same component, exactly (a plane cut: rcs_plane/pattern_plane,
+ * farfield/__init__.py:86-125); the sum over that axis is then taken once
+ * for all directions.  The caller guarantees it: direction 0's component is
+ * used for all. */
 size_t em3d_scatter_work_bytes(int nx, int ny, int nz, int batch, int ndir, int ncomp);
 int em3d_scatter_integral(int nx, int ny, int nz, const void* ax, const void* ay, const void* az, int precision,
                           int batch, int ncomp, const void* chi, int chi_kind, const void* u, int ndir,
-                          const double* dirs, double k0, double dv, double* out, void* work, hipStream_t stream);
+                          const double* dirs, int shared_axis, double k0, double dv, double* out, void* work,
+                          hipStream_t stream);
 
 /* Batch of plane waves f_b = A_b exp(i sign (k o_b . r + phi0_b)), out
  * (batch, 3, nx, ny, nz) in `precision`; ax/ay/az are the grid's cell-centre

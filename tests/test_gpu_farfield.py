@@ -91,16 +91,28 @@ def test_scatter_integral_ragged_tiles(em, prec, tol):
     for b in range(2):
         ref = ff.scatter_integral(u[b].astype(np.complex128), chi.astype(np.complex128), N, L, c, k0, dirs)
         assert _rel(F[b], ref) < tol
+    # direction sets with one component shared by all (the three plane cuts, a cone about z with
+    # d_z = 0.6, one direction) take the path that sums that axis once
+    th = np.linspace(0.0, 2.0 * np.pi, 70, endpoint=False)
+    cone = np.stack([0.8 * np.cos(th), 0.8 * np.sin(th), np.full_like(th, 0.6)], axis=-1)
+    sets = [em.farfield.plane_directions(pl, 70)[1] for pl in ("xy", "xz", "yz")] + [cone, dirs[3:4]]
+    for ds, axis in zip(sets, (2, 1, 0, 2, 2)):
+        assert em.farfield.shared_axis(ds) == axis
+        F = em.farfield.scatter_integral(be.array(u), prob, ds)
+        for b in range(2):
+            ref = ff.scatter_integral(u[b].astype(np.complex128), chi.astype(np.complex128), N, L, c, k0, ds)
+            assert _rel(F[b], ref) < tol
     from em3d_amd.acoustics import farfield as af
     N2, k2 = (2, 65, 20), 6.0
     grid2 = em.Grid(N=N2, L=(0.2, 1.0, 0.5), center=(0, 0, 0), backend=be)
     eta = (1.0 + 0.5 * (rng.standard_normal(N2) + 1j * rng.standard_normal(N2))).astype(be.complex_dtype)
     u2 = (rng.standard_normal(N2) + 1j * rng.standard_normal(N2)).astype(be.complex_dtype)
     p2 = em.acoustics.AcousticProblem(grid=grid2, eta=be.array(eta), wave=be.array(u2), k0=k2, volume=1.0)
-    amp = af.farfield_amplitude(be.array(u2), p2, dirs)
-    ref = k2 ** 2 / (4 * np.pi) * ff.scatter_integral(u2.astype(np.complex128), eta.astype(np.complex128) - 1.0,
-                                                      N2, (0.2, 1.0, 0.5), (0, 0, 0), k2, dirs)
-    assert _rel(amp, ref) < tol
+    for ds in [dirs] + sets:
+        amp = af.farfield_amplitude(be.array(u2), p2, ds)
+        ref = k2 ** 2 / (4 * np.pi) * ff.scatter_integral(u2.astype(np.complex128), eta.astype(np.complex128) - 1.0,
+                                                          N2, (0.2, 1.0, 0.5), (0, 0, 0), k2, ds)
+        assert _rel(amp, ref) < tol
 
 
 def test_acoustic_farfield_matches_oracle(em):
