@@ -293,6 +293,37 @@ def cpu_baseline(N, reps, k0):
                       f"os.cpu_count()={os.cpu_count()}"}
 
 
+def farfield_case(args):
+    """Far-field integrals over 360 directions, for profiling: acoustic 128^3
+    and 256^3 (32 inclusions, k0 = 10) and the EM ellipsoid at 128^3,
+    complex64, each for the xy plane cut (the kernel sums z once) and for 360
+    seeded random directions (default_rng(0)); warm-up + steps calls of each,
+    host array out."""
+    import em3d_amd as em
+    from em3d_amd.acoustics import farfield as af
+    from em3d_amd.acoustics.materials import eta_inclusions, random_spheres
+    be = em.Backend.hip(em.Precision.SINGLE)
+    _, plane = em.farfield.plane_directions("xy", 360)
+    rd = np.random.default_rng(0).standard_normal((360, 3))
+    rd /= np.linalg.norm(rd, axis=1, keepdims=True)
+
+    def run(name, fn):
+        for _ in range(args.warmup):
+            fn()
+        _, t = _timed(lambda: [fn() for _ in range(args.steps)])
+        log(f"[farfield] {name}: {t / args.steps * 1e3:.3f} ms per call")
+
+    for N in (128, 256):
+        grid = em.Grid(N=(N,) * 3, L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+        prob = em.acoustics.make_acoustic_problem(grid, eta_inclusions(grid, random_spheres(32, seed=3)), k0=10.0)
+        for dn, d in (("xy plane", plane), ("random", rd)):
+            run(f"acoustic {N}^3, 360 directions, {dn}", lambda: af.farfield_amplitude(prob.wave, prob, d))
+        del prob, grid
+    prob, _ = make_problem(em, be, 128, args.k0)
+    for dn, d in (("xy plane", plane), ("random", rd)):
+        run(f"EM 128^3, 360 directions, {dn}", lambda: em.farfield.scatter_integral(prob.wave, prob, d))
+
+
 def profile_case(args):
     """Matvec loop of one workload (warm-up + steps), printing its time: the
     FP64 EM matvec at N^3 (config-2 ellipsoid) or the acoustic 256^3 matvec
@@ -382,8 +413,9 @@ def main():
     ap.add_argument("--k0", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--case", default="full", choices=["full", "em-fp64", "acoustic-256", "learned-eval"],
+    ap.add_argument("--case", default="full", choices=["full", "em-fp64", "acoustic-256", "farfield", "learned-eval"],
                     help="full: the benchmark; em-fp64 / acoustic-256: only that matvec, for profiling; "
+                         "farfield: the far-field integrals, for profiling; "
                          "learned-eval: train the FNO and compare solvers on held-out materials")
     ap.add_argument("--train-materials", type=int, default=2000)
     ap.add_argument("--train-steps", type=int, default=3000)
@@ -403,6 +435,8 @@ def main():
     args = ap.parse_args()
     if args.case == "learned-eval":
         return learned_eval(args)
+    if args.case == "farfield":
+        return farfield_case(args)
     if args.case != "full":
         return profile_case(args)
     if args.gpus != 1:
