@@ -167,6 +167,7 @@ struct ZArgs {
   int perm;                    // doubled axis in mirror-interleaved order (output of a
                                // forward pass, input of an inverse pass)
   int variant;         // fast-kernel variant chosen by the plan (-1: the generic kernels)
+  int nt;              // fast kernels: rows read and written non-temporally
   const int* skip;             // device flag: when non-zero the whole pass is a no-op
 };
 
@@ -243,6 +244,7 @@ struct SArgs {
   const void* tw;
   int inv;
   int variant;         // fast-kernel variant chosen by the plan (-1: the generic kernels)
+  int nt;              // fast kernels: rows read and written non-temporally
   const int* skip;             // device flag: when non-zero the whole pass is a no-op
 };
 
@@ -298,6 +300,7 @@ struct XArgs {
   int batch;
   int perm;            // z axis of W2 in mirror-interleaved order
   int variant;         // fast-kernel variant chosen by the plan (-1: the generic kernels)
+  int nt;              // fast kernels: rows read and written non-temporally
   const int* skip;     // device flag: when non-zero the whole pass is a no-op
 };
 

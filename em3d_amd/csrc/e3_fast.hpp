@@ -207,6 +207,45 @@ E3_HD P* at_offset(P* uniform_base, unsigned lane_bytes) {
   return reinterpret_cast<P*>(reinterpret_cast<typename std::conditional<std::is_const<P>::value, const char, char>::type*>(uniform_base) + lane_bytes);
 }
 
+#ifndef E3_Z_NT
+#define E3_Z_NT 0    // 1: loads of the z passes (field, contrast) non-temporal (measured slower)
+#endif
+#ifndef E3_Z_NT_ST
+#define E3_Z_NT_ST 0 // 1: stores of the z passes non-temporal (measured slower)
+#endif
+
+// Element access, non-temporal when NT.  The x and y passes take NT when the
+// x pass's working set exceeds what the last-level caches hold (the plan's
+// nt flag): their streaming rows then do not displace what L2 keeps for
+// reuse, such as the octant run a block of the x pass reads twice.  Smaller
+// sets stay in the 256 MB Infinity Cache between passes, and plain accesses
+// are faster there.
+template <bool NT, typename T>
+E3_HD cpx<T> ld_nt(const cpx<T>* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (NT) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return mk(v.x, v.y);
+  }
+#endif
+  return *p;
+}
+template <bool NT, typename T>
+E3_HD void st_nt(cpx<T>* p, cpx<T> x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (NT) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 v;
+    v.x = x.re;
+    v.y = x.im;
+    __builtin_nontemporal_store(v, reinterpret_cast<v2*>(p));
+    return;
+  }
+#endif
+  *p = x;
+}
+
 template <typename T, bool INV>
 E3_HD cpx<T> twiddle(const cpx<T>* tw, int idx, cpx<T> x) {
   const cpx<T> w = tw[idx];
@@ -260,7 +299,7 @@ E3_FAST_CFG(double, 384, 48, 8, 8)
 // Strided pass: (Q planes) x (L rows, live n_in) x (W columns), row stride W.
 // PIN: only the first L/2 rows are live; POUT: only the first L/2 rows kept.
 // ---------------------------------------------------------------------------
-template <typename T, int L, int R1_, int TC_, bool INV, bool PIN, bool POUT>
+template <typename T, int L, int R1_, int TC_, bool INV, bool PIN, bool POUT, bool NTS = false>
 struct FastS {
   using Real = T;
   static constexpr int R1 = R1_, R2 = L / R1_, TC = TC_, NT = TC * R2, TCP = TC + 1;
@@ -286,7 +325,7 @@ struct FastS {
     const unsigned lane = (unsigned)(((long long)j * a.W + w) * sizeof(cpx<T>));
 #pragma unroll
     for (int m = 0; m < R1; ++m)
-      s.v[m] = (m < MA && ok) ? *at_offset(in + (size_t)m * R2 * a.W, lane) : czero<T>();
+      s.v[m] = (m < MA && ok) ? ld_nt<NTS>(at_offset(in + (size_t)m * R2 * a.W, lane)) : czero<T>();
     fft_reg<T, R1, INV>(s.v);
 #pragma unroll
     for (int k = 0; k < R1; ++k) lds[(k * R2 + j) * TCP + col] = s.v[k];
@@ -309,7 +348,7 @@ struct FastS {
       if (w < a.W) {
         const unsigned lane = (unsigned)(((long long)jp * a.W + w) * sizeof(cpx<T>));
 #pragma unroll
-        for (int m = 0; m < MB; ++m) *at_offset(out + (size_t)m * R1 * a.W, lane) = v[m];
+        for (int m = 0; m < MB; ++m) st_nt<NTS>(at_offset(out + (size_t)m * R1 * a.W, lane), v[m]);
       }
     }
   }
@@ -376,9 +415,8 @@ struct XR {
 #ifndef E3_X_MULGROUP
 #define E3_X_MULGROUP 2
 #endif
-
 template <typename T, int L, int C, int TC_ = 128 / (int)sizeof(cpx<T>), bool STAGE_ = true, int WAVES_ = 2,
-          class RX = XRadix<L>, bool PAIR_ = false>
+          class RX = XRadix<L>, bool PAIR_ = false, bool NTX = false>
 struct FastX {
   using Real = T;
   static constexpr int NCOMP = C;
@@ -496,7 +534,7 @@ struct FastX {
       cpx<T> v[RA];
 #pragma unroll
       for (int m = 0; m < RA; ++m)
-        v[m] = (m < RA / 2 && lv) ? *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) * sg : czero<T>();
+        v[m] = (m < RA / 2 && lv) ? ld_nt<NTX>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) * sg : czero<T>();
       fft_reg<T, RA, false>(v);
 #pragma unroll
       for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
@@ -514,7 +552,8 @@ struct FastX {
       const int jj = q.j + bb * NTC;
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
 #pragma unroll
-      for (int m = 0; m < RA / 2; ++m) s.pre[bb][m] = lv ? *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) : czero<T>();
+      for (int m = 0; m < RA / 2; ++m)
+        s.pre[bb][m] = lv ? ld_nt<NTX>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) : czero<T>();
     }
   }
   E3_HD void fwd_a_pre(int c, int tid, int h, St& s, int buf) const {
@@ -798,7 +837,7 @@ struct FastX {
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
       const T sg = comp_sign(c, q);
 #pragma unroll
-      for (int m = 0; m < RA / 2; ++m) *at_offset(d + (size_t)m * (L / RA) * a.W, lane_b) = v[m] * sg;
+      for (int m = 0; m < RA / 2; ++m) st_nt<NTX>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b), v[m] * sg);
     }
   }
 };
@@ -809,15 +848,16 @@ struct FastX {
 // ---------------------------------------------------------------------------
 template <typename T>
 E3_HD cpx<T> contrast_row(const ZArgs& a, const cpx<T>* in, int c, long long p, int n) {
-  const cpx<T> uc = in[(size_t)c * a.in_cs + (size_t)p * a.in_ps + n];
+  const cpx<T> uc = ld_nt<E3_Z_NT>(&in[(size_t)c * a.in_cs + (size_t)p * a.in_ps + n]);
   if (a.pro != PRO_CHI || a.chi_kind == CHI_NONE) return uc;
   const cpx<T>* chi = (const cpx<T>*)a.chi;
   const long long cell = p * a.n_in + n;
-  if (a.chi_kind == CHI_ISO) return chi[cell] * uc;
-  if (a.chi_kind == CHI_DIAG) return chi[(size_t)c * a.ncell + cell] * uc;
+  if (a.chi_kind == CHI_ISO) return ld_nt<E3_Z_NT>(&chi[cell]) * uc;
+  if (a.chi_kind == CHI_DIAG) return ld_nt<E3_Z_NT>(&chi[(size_t)c * a.ncell + cell]) * uc;
   cpx<T> acc = czero<T>();
   for (int k = 0; k < 3; ++k)
-    acc = acc + chi[(size_t)(c * 3 + k) * a.ncell + cell] * in[(size_t)k * a.in_cs + (size_t)p * a.in_ps + n];
+    acc = acc + ld_nt<E3_Z_NT>(&chi[(size_t)(c * 3 + k) * a.ncell + cell]) *
+                    ld_nt<E3_Z_NT>(&in[(size_t)k * a.in_cs + (size_t)p * a.in_ps + n]);
   return acc;
 }
 
@@ -882,7 +922,8 @@ struct FastZF {
       fft_reg<T, R2, false>(v);
       if (p < a.P) {
 #pragma unroll
-        for (int m = 0; m < R2; ++m) out[a.perm ? mirror_position(jp + m * R1, L / 2, L) : jp + m * R1] = v[m];
+        for (int m = 0; m < R2; ++m)
+          st_nt<E3_Z_NT_ST>(&out[a.perm ? mirror_position(jp + m * R1, L / 2, L) : jp + m * R1], v[m]);
       }
     }
   }
@@ -923,7 +964,7 @@ struct FastZI {
 #pragma unroll
     for (int m = 0; m < R1; ++m) {
       const int n = j + m * R2;
-      v[m] = ok ? in[a.perm ? mirror_position(n, L / 2, L) : n] : czero<T>();
+      v[m] = ok ? ld_nt<E3_Z_NT>(&in[a.perm ? mirror_position(n, L / 2, L) : n]) : czero<T>();
     }
   }
   E3_HD void fft_a(int c, int tid, cpx<T>* v) const {
@@ -979,10 +1020,10 @@ struct FastZI {
         cpx<T> t = s.y[0][bb][m];
         if (herm) {
           const long long cell = p * a.n_out + z;
-          t = cmulc(a.chi_kind == CHI_ISO ? chi[cell] : chi[(size_t)c * a.ncell + cell], t);
+          t = cmulc(ld_nt<E3_Z_NT>(a.chi_kind == CHI_ISO ? &chi[cell] : &chi[(size_t)c * a.ncell + cell]), t);
         }
         const size_t o = (size_t)c * a.out_cs + (size_t)p * a.out_ps + z;
-        out[o] = src[o] - t;
+        st_nt<E3_Z_NT_ST>(&out[o], ld_nt<E3_Z_NT>(&src[o]) - t);
       }
     }
   }
@@ -1012,7 +1053,7 @@ struct FastZI {
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const size_t o = (size_t)c * a.out_cs + (size_t)p * a.out_ps + z;
-          out[o] = src[o] - t[c];
+          st_nt<E3_Z_NT_ST>(&out[o], ld_nt<E3_Z_NT>(&src[o]) - t[c]);
         }
       }
     }
@@ -1045,33 +1086,33 @@ template <> struct SVar<float, 256, 2> { static constexpr int R1 = 16, TC = 32; 
 template <> struct SVar<float, 192, 1> { static constexpr int R1 = 48, TC = 32; };
 template <> struct SVar<float, 192, 2> { static constexpr int R1 = 24, TC = 16; };
 
-template <typename T, int L, int V, class Run>
+template <typename T, int L, int V, bool NT, class Run>
 bool fast_s_var(Run& run, const SArgs& a) {
   using Cfg = SVar<T, L, V>;
   if (a.n_in * 2 == L && a.n_out == L && !a.inv) {
-    run.template s<FastS<T, L, Cfg::R1, Cfg::TC, false, true, false>>(a);
+    run.template s<FastS<T, L, Cfg::R1, Cfg::TC, false, true, false, NT>>(a);
     return true;
   }
   if (a.n_in == L && a.n_out * 2 == L && a.inv) {
-    run.template s<FastS<T, L, Cfg::R1, Cfg::TC, true, false, true>>(a);
+    run.template s<FastS<T, L, Cfg::R1, Cfg::TC, true, false, true, NT>>(a);
     return true;
   }
   if constexpr (sizeof(T) == 8) {
     if (a.n_in == L && a.n_out == L && !a.inv) {
-      run.template s<FastS<T, L, Cfg::R1, Cfg::TC, false, false, false>>(a);
+      run.template s<FastS<T, L, Cfg::R1, Cfg::TC, false, false, false, NT>>(a);
       return true;
     }
   }
   return false;
 }
 
-template <typename T, int L, class Run>
+template <typename T, int L, bool NT, class Run>
 bool fast_s_len(Run& run, const SArgs& a) {
   if constexpr (std::is_same<T, float>::value && (L == 256 || L == 192)) {
-    if (a.variant == 1) return fast_s_var<T, L, 1>(run, a);
-    if (a.variant == 2) return fast_s_var<T, L, 2>(run, a);
+    if (a.variant == 1) return fast_s_var<T, L, 1, NT>(run, a);
+    if (a.variant == 2) return fast_s_var<T, L, 2, NT>(run, a);
   }
-  return fast_s_var<T, L, 0>(run, a);
+  return fast_s_var<T, L, 0, NT>(run, a);
 }
 
 // x-pass tile of the 3-component operator: columns per block, octant run
@@ -1086,11 +1127,11 @@ bool fast_s_len(Run& run, const SArgs& a) {
 #define E3_X3_WAVES 2
 #endif
 
-template <typename T, int L, class RX, int TC, class Run>
+template <typename T, int L, class RX, int TC, bool NT, class Run>
 bool fast_x3(Run& run, const XArgs& a) {
   // mixed-radix lengths hold more live values: one wave per SIMD keeps them
   // in registers (VGPR + AGPR) instead of scratch
-  using X3 = FastX<T, L, 3, TC, E3_X3_STAGE, is_pow2(L) ? E3_X3_WAVES : 1, RX>;
+  using X3 = FastX<T, L, 3, TC, E3_X3_STAGE, is_pow2(L) ? E3_X3_WAVES : 1, RX, false, NT>;
   if (a.L[2] % X3::TC != 0 || X3::NT > 1024) return false;
   run.template x<X3>(a);
   return true;
@@ -1107,26 +1148,26 @@ bool fast_x3(Run& run, const XArgs& a) {
 #define E3_X1_PAIR 1   // both rows of a ky pair in one schedule
 #endif
 
-template <typename T, int L, class Run>
+template <typename T, int L, bool NT, class Run>
 bool fast_x_len(Run& run, const XArgs& a) {
-  using X1 = FastX<T, L, 1, E3_X1_TC, E3_X1_STAGE, 2, XRadix<L>, E3_X1_PAIR>;
+  using X1 = FastX<T, L, 1, E3_X1_TC, E3_X1_STAGE, 2, XRadix<L>, E3_X1_PAIR, NT>;
   if (a.N[0] * 2 != L || !a.perm) return false;
   if (a.C == 3) {
     constexpr int TCd = E3_X3_TC;
     if constexpr (std::is_same<T, float>::value && L == 256) {
-      if (a.variant == 1) return fast_x3<T, L, XR<4, 4, 16>, TCd>(run, a);
-      if (a.variant == 2) return fast_x3<T, L, XR<8, 4, 8>, 8>(run, a);
-      if (a.variant == 3) return fast_x3<T, L, XR<16, 1, 16>, TCd>(run, a);
+      if (a.variant == 1) return fast_x3<T, L, XR<4, 4, 16>, TCd, NT>(run, a);
+      if (a.variant == 2) return fast_x3<T, L, XR<8, 4, 8>, 8, NT>(run, a);
+      if (a.variant == 3) return fast_x3<T, L, XR<16, 1, 16>, TCd, NT>(run, a);
     }
     if constexpr (std::is_same<T, float>::value && L == 128) {
-      if (a.variant == 1) return fast_x3<T, L, XR<8, 2, 8>, TCd>(run, a);
-      if (a.variant == 2) return fast_x3<T, L, XR<8, 1, 16>, TCd>(run, a);
+      if (a.variant == 1) return fast_x3<T, L, XR<8, 2, 8>, TCd, NT>(run, a);
+      if (a.variant == 2) return fast_x3<T, L, XR<8, 1, 16>, TCd, NT>(run, a);
     }
     if constexpr (std::is_same<T, float>::value && L == 192) {
-      if (a.variant == 1) return fast_x3<T, L, XR<4, 2, 24>, TCd>(run, a);
-      if (a.variant == 2) return fast_x3<T, L, XR<8, 1, 24>, TCd>(run, a);
+      if (a.variant == 1) return fast_x3<T, L, XR<4, 2, 24>, TCd, NT>(run, a);
+      if (a.variant == 2) return fast_x3<T, L, XR<8, 1, 24>, TCd, NT>(run, a);
     }
-    return fast_x3<T, L, XRadix<L>, TCd>(run, a);
+    return fast_x3<T, L, XRadix<L>, TCd, NT>(run, a);
   }
   if (a.L[2] % X1::TC != 0 || X1::NT > 1024) return false;
   run.template x<X1>(a);
@@ -1168,7 +1209,7 @@ bool fast_z_var(Run& run, const ZArgs& a) {
   return false;
 }
 
-template <typename T, int L, class Run>
+template <typename T, int L, bool NT, class Run>
 bool fast_z_len(Run& run, const ZArgs& a) {
   if constexpr (std::is_same<T, float>::value && (L == 256 || L == 192)) {
     if (a.variant == 1) return fast_z_var<T, L, 1>(run, a);
@@ -1178,46 +1219,48 @@ bool fast_z_len(Run& run, const ZArgs& a) {
 
 #define E3_FAST_SWITCH(FN, T, RUN, ARGS)        \
   switch (ARGS.plan.L) {                        \
-    case 16: return FN<T, 16>(RUN, ARGS);       \
-    case 32: return FN<T, 32>(RUN, ARGS);       \
-    case 64: return FN<T, 64>(RUN, ARGS);       \
-    case 128: return FN<T, 128>(RUN, ARGS);     \
-    case 256: return FN<T, 256>(RUN, ARGS);     \
-    case 512: return FN<T, 512>(RUN, ARGS);     \
+    case 16: return FN<T, 16, NT>(RUN, ARGS);       \
+    case 32: return FN<T, 32, NT>(RUN, ARGS);       \
+    case 64: return FN<T, 64, NT>(RUN, ARGS);       \
+    case 128: return FN<T, 128, NT>(RUN, ARGS);     \
+    case 256: return FN<T, 256, NT>(RUN, ARGS);     \
+    case 512: return FN<T, 512, NT>(RUN, ARGS);     \
     default: return false;                      \
   }
 
 // strided and z passes also cover 3 and 5 times a power of two
 #define E3_FAST_SWITCH_MIXED(FN, T, RUN, ARGS)  \
   switch (ARGS.plan.L) {                        \
-    case 16: return FN<T, 16>(RUN, ARGS);       \
-    case 32: return FN<T, 32>(RUN, ARGS);       \
-    case 48: return FN<T, 48>(RUN, ARGS);       \
-    case 64: return FN<T, 64>(RUN, ARGS);       \
-    case 80: return FN<T, 80>(RUN, ARGS);       \
-    case 96: return FN<T, 96>(RUN, ARGS);       \
-    case 128: return FN<T, 128>(RUN, ARGS);     \
-    case 160: return FN<T, 160>(RUN, ARGS);     \
-    case 192: return FN<T, 192>(RUN, ARGS);     \
-    case 256: return FN<T, 256>(RUN, ARGS);     \
-    case 320: return FN<T, 320>(RUN, ARGS);     \
-    case 384: return FN<T, 384>(RUN, ARGS);     \
-    case 512: return FN<T, 512>(RUN, ARGS);     \
+    case 16: return FN<T, 16, NT>(RUN, ARGS);       \
+    case 32: return FN<T, 32, NT>(RUN, ARGS);       \
+    case 48: return FN<T, 48, NT>(RUN, ARGS);       \
+    case 64: return FN<T, 64, NT>(RUN, ARGS);       \
+    case 80: return FN<T, 80, NT>(RUN, ARGS);       \
+    case 96: return FN<T, 96, NT>(RUN, ARGS);       \
+    case 128: return FN<T, 128, NT>(RUN, ARGS);     \
+    case 160: return FN<T, 160, NT>(RUN, ARGS);     \
+    case 192: return FN<T, 192, NT>(RUN, ARGS);     \
+    case 256: return FN<T, 256, NT>(RUN, ARGS);     \
+    case 320: return FN<T, 320, NT>(RUN, ARGS);     \
+    case 384: return FN<T, 384, NT>(RUN, ARGS);     \
+    case 512: return FN<T, 512, NT>(RUN, ARGS);     \
     default: return false;                      \
   }
 
-template <typename T, class Run>
+// NT: the non-temporal instantiations of the x and y passes (SArgs/XArgs nt)
+template <typename T, bool NT = false, class Run>
 bool fast_s(Run& run, const SArgs& a) {
   if (a.variant < 0) return false;
   E3_FAST_SWITCH_MIXED(fast_s_len, T, run, a)
 }
-template <typename T, class Run>
+template <typename T, bool NT = false, class Run>
 bool fast_x(Run& run, const XArgs& a) {
   if (a.variant < 0) return false;
   E3_FAST_SWITCH_MIXED(fast_x_len, T, run, a)
 }
 template <typename T, class Run>
 bool fast_z(Run& run, const ZArgs& a) {
+  constexpr bool NT = false;
   if (a.variant < 0) return false;
   E3_FAST_SWITCH_MIXED(fast_z_len, T, run, a)
 }

@@ -4,6 +4,7 @@
 namespace e3 {
 
 bool launch_fast_s_f32(const SArgs& a, hipStream_t st, hipError_t* err) {
+  if (a.nt) return launch_fast_s_f32_nt(a, st, err);
   HipFastRun r{st};
   const bool ok = fast_s<float>(r, a);
   *err = r.err;
@@ -11,6 +12,7 @@ bool launch_fast_s_f32(const SArgs& a, hipStream_t st, hipError_t* err) {
 }
 
 bool launch_fast_x_f32(const XArgs& a, hipStream_t st, hipError_t* err) {
+  if (a.nt) return launch_fast_x_f32_nt(a, st, err);
   HipFastRun r{st};
   const bool ok = fast_x<float>(r, a);
   *err = r.err;

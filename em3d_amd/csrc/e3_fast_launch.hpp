@@ -122,5 +122,10 @@ bool launch_fast_z_f32(const ZArgs& a, hipStream_t st, hipError_t* err);
 bool launch_fast_s_f64(const SArgs& a, hipStream_t st, hipError_t* err);
 bool launch_fast_x_f64(const XArgs& a, hipStream_t st, hipError_t* err);
 bool launch_fast_z_f64(const ZArgs& a, hipStream_t st, hipError_t* err);
+// Defined in e3_fast_f32_nt.hip / e3_fast_f64_nt.hip (non-temporal rows).
+bool launch_fast_s_f32_nt(const SArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_x_f32_nt(const XArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_s_f64_nt(const SArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_x_f64_nt(const XArgs& a, hipStream_t st, hipError_t* err);
 
 }  // namespace e3

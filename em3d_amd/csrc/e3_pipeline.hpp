@@ -117,6 +117,8 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   const void* twz = prec == PREC_DOUBLE ? p.tw_d[2] : p.tw_f[2];
   const void* twy = prec == PREC_DOUBLE ? p.tw_d[1] : p.tw_f[1];
   const void* twx = prec == PREC_DOUBLE ? p.tw_d[0] : p.tw_f[0];
+  // the x pass's working set (W2) beyond 128 MiB: the y and x passes stream it non-temporally
+  const int nt = (double)p.batch * C * Nx * Ly * Lz * es > 128.0 * 1024 * 1024 ? 1 : 0;
 
   // P1: z forward, chi prologue (matvec only), zero half skipped
   ZArgs z;
@@ -134,7 +136,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   std::memset(&s, 0, sizeof(s));
   s.in = W1; s.out = W2; s.Q = (long long)p.batch * C * Nx; s.W = Lz;
   s.in_qs = (long long)Ny * Lz; s.out_qs = (long long)Ly * Lz;
-  s.n_in = Ny; s.n_out = Ly; s.inv = 0; s.skip = skip; s.variant = p.variant[PASS_S];
+  s.n_in = Ny; s.n_out = Ly; s.inv = 0; s.skip = skip; s.variant = p.variant[PASS_S]; s.nt = nt;
   s_pass(go, prec, twy, p.ax[1], s);
 
   // P3: x forward, spectral multiply, x inverse (Nx rows kept), in place
@@ -144,6 +146,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   for (int a = 0; a < 3; ++a) { x.N[a] = p.N[a]; x.L[a] = p.L[a]; }
   x.plan = p.ax[0]; x.tw = twx; x.khat = khat; x.adj = adjoint ? 1 : 0; x.batch = p.batch; x.perm = 1; x.skip = skip;
   x.variant = p.variant[PASS_X];
+  x.nt = nt;
   x.ps = Lx + 1;
   x.T = tile_count(lds_bytes(p.C, x.ps, prec), 16);
   if (x.T > x.W) x.T = (int)x.W;
