@@ -207,6 +207,24 @@ E3_HD P* at_offset(P* uniform_base, unsigned lane_bytes) {
   return reinterpret_cast<P*>(reinterpret_cast<typename std::conditional<std::is_const<P>::value, const char, char>::type*>(uniform_base) + lane_bytes);
 }
 
+#ifndef E3_X_LD_NT
+#define E3_X_LD_NT 1   // with a plan's nt: x-pass row loads non-temporal
+#endif
+#ifndef E3_X_ST_NT
+#define E3_X_ST_NT 1   // with a plan's nt: x-pass row stores non-temporal
+#endif
+#ifndef E3_SF_LD_NT
+#define E3_SF_LD_NT 1  // with a plan's nt: forward y-pass loads / stores non-temporal
+#endif
+#ifndef E3_SF_ST_NT
+#define E3_SF_ST_NT 1
+#endif
+#ifndef E3_SI_LD_NT
+#define E3_SI_LD_NT 1  // with a plan's nt: inverse y-pass loads / stores non-temporal
+#endif
+#ifndef E3_SI_ST_NT
+#define E3_SI_ST_NT 1
+#endif
 #ifndef E3_Z_NT
 #define E3_Z_NT 0    // 1: loads of the z passes (field, contrast) non-temporal (measured slower)
 #endif
@@ -325,7 +343,7 @@ struct FastS {
     const unsigned lane = (unsigned)(((long long)j * a.W + w) * sizeof(cpx<T>));
 #pragma unroll
     for (int m = 0; m < R1; ++m)
-      s.v[m] = (m < MA && ok) ? ld_nt<NTS>(at_offset(in + (size_t)m * R2 * a.W, lane)) : czero<T>();
+      s.v[m] = (m < MA && ok) ? ld_nt<NTS && (INV ? E3_SI_LD_NT : E3_SF_LD_NT)>(at_offset(in + (size_t)m * R2 * a.W, lane)) : czero<T>();
     fft_reg<T, R1, INV>(s.v);
 #pragma unroll
     for (int k = 0; k < R1; ++k) lds[(k * R2 + j) * TCP + col] = s.v[k];
@@ -348,7 +366,7 @@ struct FastS {
       if (w < a.W) {
         const unsigned lane = (unsigned)(((long long)jp * a.W + w) * sizeof(cpx<T>));
 #pragma unroll
-        for (int m = 0; m < MB; ++m) st_nt<NTS>(at_offset(out + (size_t)m * R1 * a.W, lane), v[m]);
+        for (int m = 0; m < MB; ++m) st_nt<NTS && (INV ? E3_SI_ST_NT : E3_SF_ST_NT)>(at_offset(out + (size_t)m * R1 * a.W, lane), v[m]);
       }
     }
   }
@@ -534,7 +552,7 @@ struct FastX {
       cpx<T> v[RA];
 #pragma unroll
       for (int m = 0; m < RA; ++m)
-        v[m] = (m < RA / 2 && lv) ? ld_nt<NTX>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) * sg : czero<T>();
+        v[m] = (m < RA / 2 && lv) ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) * sg : czero<T>();
       fft_reg<T, RA, false>(v);
 #pragma unroll
       for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
@@ -553,7 +571,7 @@ struct FastX {
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
 #pragma unroll
       for (int m = 0; m < RA / 2; ++m)
-        s.pre[bb][m] = lv ? ld_nt<NTX>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) : czero<T>();
+        s.pre[bb][m] = lv ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) : czero<T>();
     }
   }
   E3_HD void fwd_a_pre(int c, int tid, int h, St& s, int buf) const {
@@ -837,7 +855,7 @@ struct FastX {
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
       const T sg = comp_sign(c, q);
 #pragma unroll
-      for (int m = 0; m < RA / 2; ++m) st_nt<NTX>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b), v[m] * sg);
+      for (int m = 0; m < RA / 2; ++m) st_nt<NTX && E3_X_ST_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b), v[m] * sg);
     }
   }
 };
