@@ -225,8 +225,14 @@ E3_HD P* at_offset(P* uniform_base, unsigned lane_bytes) {
 #ifndef E3_SI_ST_NT
 #define E3_SI_ST_NT 1
 #endif
+#ifndef E3_ZF_NT_U
+#define E3_ZF_NT_U 1    // with a plan's nt: field loads of the forward z pass non-temporal
+#endif
+#ifndef E3_ZF_NT_CHI
+#define E3_ZF_NT_CHI 1  // with a plan's nt: contrast loads of the forward z pass non-temporal
+#endif
 #ifndef E3_Z_NT
-#define E3_Z_NT 0    // 1: loads of the z passes (field, contrast) non-temporal (measured slower)
+#define E3_Z_NT 0    // 1: loads of the inverse z pass (field, contrast) non-temporal
 #endif
 #ifndef E3_Z_NT_ST
 #define E3_Z_NT_ST 0 // 1: stores of the z passes non-temporal (measured slower)
@@ -864,22 +870,25 @@ struct FastX {
 // z pass, forward: pencils of the contiguous axis, components one after the
 // other.  PIN: n_in = L/2 with the contrast prologue; else a full transform.
 // ---------------------------------------------------------------------------
-template <typename T>
+// NTZ: contrast loads non-temporal; NTU: field loads too (the scalar operator:
+// a 3-component epilogue re-reads u, which plain loads leave in the caches)
+template <typename T, bool NTZ = false, bool NTU = false>
 E3_HD cpx<T> contrast_row(const ZArgs& a, const cpx<T>* in, int c, long long p, int n) {
-  const cpx<T> uc = ld_nt<E3_Z_NT>(&in[(size_t)c * a.in_cs + (size_t)p * a.in_ps + n]);
+  constexpr bool NU = NTU && E3_ZF_NT_U, NC = NTZ && E3_ZF_NT_CHI;
+  const cpx<T> uc = ld_nt<NU>(&in[(size_t)c * a.in_cs + (size_t)p * a.in_ps + n]);
   if (a.pro != PRO_CHI || a.chi_kind == CHI_NONE) return uc;
   const cpx<T>* chi = (const cpx<T>*)a.chi;
   const long long cell = p * a.n_in + n;
-  if (a.chi_kind == CHI_ISO) return ld_nt<E3_Z_NT>(&chi[cell]) * uc;
-  if (a.chi_kind == CHI_DIAG) return ld_nt<E3_Z_NT>(&chi[(size_t)c * a.ncell + cell]) * uc;
+  if (a.chi_kind == CHI_ISO) return ld_nt<NC>(&chi[cell]) * uc;
+  if (a.chi_kind == CHI_DIAG) return ld_nt<NC>(&chi[(size_t)c * a.ncell + cell]) * uc;
   cpx<T> acc = czero<T>();
   for (int k = 0; k < 3; ++k)
-    acc = acc + ld_nt<E3_Z_NT>(&chi[(size_t)(c * 3 + k) * a.ncell + cell]) *
-                    ld_nt<E3_Z_NT>(&in[(size_t)k * a.in_cs + (size_t)p * a.in_ps + n]);
+    acc = acc + ld_nt<NC>(&chi[(size_t)(c * 3 + k) * a.ncell + cell]) *
+                    ld_nt<NU>(&in[(size_t)k * a.in_cs + (size_t)p * a.in_ps + n]);
   return acc;
 }
 
-template <typename T, int L, int R1_, int P_, bool PIN>
+template <typename T, int L, int R1_, int P_, bool PIN, bool NTZ = false, bool NTU = false>
 struct FastZF {
   using Real = T;
   static constexpr int R1 = R1_, R2 = L / R1_, P = P_, NT = P * R2, R2P = R2 + 1;
@@ -906,7 +915,7 @@ struct FastZF {
     const bool ok = p < a.P;
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs;
 #pragma unroll
-    for (int m = 0; m < R1; ++m) v[m] = (m < MA && ok) ? contrast_row<T>(a, in, c, p, j + m * R2) : czero<T>();
+    for (int m = 0; m < R1; ++m) v[m] = (m < MA && ok) ? contrast_row<T, NTZ, NTU>(a, in, c, p, j + m * R2) : czero<T>();
   }
   E3_HD void fft_a(int tid, cpx<T>* v) const {
     int j = tid % R2, pl = tid / R2;
@@ -1200,14 +1209,18 @@ struct ZVar {
 template <> struct ZVar<float, 256, 1> { static constexpr int R1 = 32, P = 32; };
 template <> struct ZVar<float, 192, 1> { static constexpr int R1 = 48, P = 32; };
 
-template <typename T, int L, int V, class Run>
+template <typename T, int L, int V, bool NT, class Run>
 bool fast_z_var(Run& run, const ZArgs& a) {
   using Cfg = ZVar<T, L, V>;
   if (a.C != 1 && a.C != 3) return false;
   if (a.n_in * 2 == L && a.n_out == L && a.inv == 0 && a.epi == EPI_STORE) {
-    run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, true>>(a);
+    if (NT && a.C == 1)
+      run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, true, NT, NT>>(a);
+    else
+      run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, true, NT, false>>(a);
     return true;
   }
+  if constexpr (NT) return false;   // the non-temporal set holds the forward pass only
   if (a.n_in == L && a.n_out * 2 == L && a.inv == 1 && a.pro == PRO_COPY &&
       (a.epi == EPI_SUB || a.epi == EPI_SUB_CHIH)) {
     if (a.C == 3 && a.epi == EPI_SUB_CHIH && a.chi_kind == CHI_FULL)
@@ -1230,9 +1243,9 @@ bool fast_z_var(Run& run, const ZArgs& a) {
 template <typename T, int L, bool NT, class Run>
 bool fast_z_len(Run& run, const ZArgs& a) {
   if constexpr (std::is_same<T, float>::value && (L == 256 || L == 192)) {
-    if (a.variant == 1) return fast_z_var<T, L, 1>(run, a);
+    if (a.variant == 1) return fast_z_var<T, L, 1, NT>(run, a);
   }
-  return fast_z_var<T, L, 0>(run, a);
+  return fast_z_var<T, L, 0, NT>(run, a);
 }
 
 #define E3_FAST_SWITCH(FN, T, RUN, ARGS)        \
@@ -1265,7 +1278,7 @@ bool fast_z_len(Run& run, const ZArgs& a) {
     default: return false;                      \
   }
 
-// NT: the non-temporal instantiations of the x and y passes (SArgs/XArgs nt)
+// NT: the non-temporal instantiations of the y, x and forward z passes (the args' nt)
 template <typename T, bool NT = false, class Run>
 bool fast_s(Run& run, const SArgs& a) {
   if (a.variant < 0) return false;
@@ -1276,9 +1289,8 @@ bool fast_x(Run& run, const XArgs& a) {
   if (a.variant < 0) return false;
   E3_FAST_SWITCH_MIXED(fast_x_len, T, run, a)
 }
-template <typename T, class Run>
+template <typename T, bool NT = false, class Run>
 bool fast_z(Run& run, const ZArgs& a) {
-  constexpr bool NT = false;
   if (a.variant < 0) return false;
   E3_FAST_SWITCH_MIXED(fast_z_len, T, run, a)
 }

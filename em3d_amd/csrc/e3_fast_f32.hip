@@ -20,6 +20,7 @@ bool launch_fast_x_f32(const XArgs& a, hipStream_t st, hipError_t* err) {
 }
 
 bool launch_fast_z_f32(const ZArgs& a, hipStream_t st, hipError_t* err) {
+  if (a.nt) return launch_fast_z_f32_nt(a, st, err);
   HipFastRun r{st};
   const bool ok = fast_z<float>(r, a);
   *err = r.err;

@@ -127,5 +127,7 @@ bool launch_fast_s_f32_nt(const SArgs& a, hipStream_t st, hipError_t* err);
 bool launch_fast_x_f32_nt(const XArgs& a, hipStream_t st, hipError_t* err);
 bool launch_fast_s_f64_nt(const SArgs& a, hipStream_t st, hipError_t* err);
 bool launch_fast_x_f64_nt(const XArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_z_f32_nt(const ZArgs& a, hipStream_t st, hipError_t* err);
+bool launch_fast_z_f64_nt(const ZArgs& a, hipStream_t st, hipError_t* err);
 
 }  // namespace e3

@@ -117,7 +117,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   const void* twz = prec == PREC_DOUBLE ? p.tw_d[2] : p.tw_f[2];
   const void* twy = prec == PREC_DOUBLE ? p.tw_d[1] : p.tw_f[1];
   const void* twx = prec == PREC_DOUBLE ? p.tw_d[0] : p.tw_f[0];
-  // the x pass's working set (W2) beyond 128 MiB: the y and x passes stream it non-temporally
+  // the x pass's working set (W2) beyond 128 MiB: the forward z, y and x passes stream non-temporally
   const int nt = (double)p.batch * C * Nx * Ly * Lz * es > 128.0 * 1024 * 1024 ? 1 : 0;
 
   // P1: z forward, chi prologue (matvec only), zero half skipped
@@ -129,6 +129,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.n_in = Nz; z.n_out = Lz; z.inv = 0;
   z.pro = adjoint ? PRO_COPY : PRO_CHI; z.epi = EPI_STORE;
   z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.perm = 1; z.skip = skip; z.variant = p.variant[PASS_Z];
+  z.nt = nt;
   z_pass(go, p, prec, twz, p.ax[2], z);
 
   // P2: y forward (Ny live rows -> Ly)
