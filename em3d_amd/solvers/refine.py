@@ -25,8 +25,8 @@ Each outer step divides the residual by about ``inner_rtol``, so rtol 1e-8
 takes two or three steps.  When a step fails to reduce the true residual by
 ``stagnation`` (an inner breakdown, or an operator whose complex64
 conditioning caps the attainable accuracy), the remaining correction is
-solved by plain FP64 BiCGStab, which therefore stays both the fallback and,
-through the FP64 residual, the judge.
+solved by plain FP64 BiCGStab.  That solver is the fallback, and the FP64
+true residual decides convergence in every case.
 """
 from __future__ import annotations
 
