@@ -81,8 +81,9 @@ __device__ __forceinline__ cpx<double> pol_at(const cpx<T>* chi, int chi_kind, c
 // Tiles of the factored sum F_d = sum_x X_d(x) sum_y Y_d(y) sum_z Z_d(z) p(x, y, z).
 // A block takes one x plane, one tile of kDT directions and one field.  The
 // z sum is a complex product (y rows x z) . (z x directions) staged through
-// LDS in chunks of kKZ, each thread holding a kFR x kFD tile of it in FP64
-// registers; the y sum and the X factor follow in the block's epilogue.
+// LDS in chunks of KZ: on the FP64 MFMA, each wave holding a 32 x 32 block
+// of it (E3_FF_MFMA), or as FP64 FMAs with each thread holding kFR x kFD.
+// The y sum and the X factor follow in the block's epilogue.
 constexpr int kFR = 4, kFD = 4;         // rows (y) x directions per thread
 constexpr int kRT = 16 * kFR;           // y rows per tile
 constexpr int kDT = 16 * kFD;           // directions per tile
@@ -104,7 +105,6 @@ constexpr int kDT = 16 * kFD;           // directions per tile
 constexpr int kKZ = E3_FF_KZ;
 static_assert(kKZ % 4 == 0 && kRT == 64 && kDT == 64 && kT == 256, "tile shape of the far-field product");
 typedef double d4v __attribute__((ext_vector_type(4)));
-constexpr int kKP = kKZ + 1;            // padded LDS row
 
 // grid (nx, direction tiles, batch); part[b][m][a][x] (complex).  The block
 // walks chunks ch = (component a, y tile, z chunk) in that order; a chunk's
