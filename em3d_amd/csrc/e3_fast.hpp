@@ -213,6 +213,9 @@ E3_HD P* at_offset(P* uniform_base, unsigned lane_bytes) {
 #ifndef E3_X_ST_NT
 #define E3_X_ST_NT 1   // with a plan's nt: x-pass row stores non-temporal
 #endif
+#ifndef E3_X_K_NT
+#define E3_X_K_NT 0    // with a plan's nt: octant loads of the x pass non-temporal
+#endif
 #ifndef E3_SF_LD_NT
 #define E3_SF_LD_NT 1  // with a plan's nt: forward y-pass loads / stores non-temporal
 #endif
@@ -768,7 +771,7 @@ struct FastX {
   static constexpr int KSTR = STAGE ? S + 1 : S;   // element stride between components
   E3_HD cpx<T> kload(const cpx<T>* base, int off) const {
     if constexpr (STAGE) return base[off];
-    return *at_offset(base, (unsigned)off * (unsigned)sizeof(cpx<T>));
+    return ld_nt<NTX && E3_X_K_NT>(at_offset(base, (unsigned)off * (unsigned)sizeof(cpx<T>)));
   }
   // element of (ix = 0, comp = 0) of this lane's octant column
   E3_HD int kcol(const Lane& q) const {
