@@ -15,23 +15,31 @@ from . import _lib
 from .operator import pack_contrast, require_hip
 
 _PACKED: dict = {}
+_SELF = object()   # stands for the key tensor inside a cached value
 
 
 def packed_contrast(t, make):
     """make(t), kept while t is alive and unchanged: the same tensor object
     with the same version counter (torch bumps it on every in-place change).
     The entry is dropped when t is freed.  Repeated far-field calls on one
-    problem (a sweep of planes, of fields) then pack the contrast once."""
+    problem (a sweep of planes, of fields) then pack the contrast once.  A
+    value that is t itself, or a tuple holding it (a full 3 x 3 contrast
+    packs to itself), is stored without that reference, so the cache never
+    keeps t alive."""
     if not hasattr(t, "_version"):
         return make(t)
     key = id(t)
     hit = _PACKED.get(key)
     if hit is not None and hit[0]() is t and hit[1] == t._version:
-        return hit[2]
+        v = hit[2]
+        if isinstance(v, tuple):
+            return tuple(t if x is _SELF else x for x in v)
+        return t if v is _SELF else v
     val = make(t)
     if hit is None:
         weakref.finalize(t, _PACKED.pop, key, None)
-    _PACKED[key] = (weakref.ref(t), t._version, val)
+    kept = tuple(_SELF if x is t else x for x in val) if isinstance(val, tuple) else (_SELF if val is t else val)
+    _PACKED[key] = (weakref.ref(t), t._version, kept)
     return val
 
 __all__ = ["scatter_integral", "scatter_integral_device", "rcs", "rcs_plane", "plane_directions"]

@@ -45,3 +45,17 @@ def test_packed_contrast_is_reused_until_the_tensor_changes():
     gc.collect()
     assert key not in ff._PACKED          # dropped with its tensor
     assert ff.packed_contrast(np.ones(3), lambda x: x + 1).tolist() == [2.0, 2.0, 2.0]
+
+
+def test_packed_contrast_never_keeps_its_tensor_alive():
+    """A full 3 x 3 contrast packs to itself: the cached value must not hold it."""
+    from em3d_amd.operator import pack_contrast, CHI_FULL
+    t = torch.randn(3, 3, 4, 4, 4, dtype=torch.complex64)
+    kind, chi = ff.packed_contrast(t, pack_contrast)
+    assert kind == CHI_FULL and chi is t
+    kind2, chi2 = ff.packed_contrast(t, pack_contrast)     # from the cache
+    assert kind2 == CHI_FULL and chi2 is t
+    key = id(t)
+    del t, chi, chi2
+    gc.collect()
+    assert key not in ff._PACKED
