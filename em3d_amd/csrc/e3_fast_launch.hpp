@@ -49,33 +49,39 @@ __global__ __launch_bounds__(K::NT, K::MIN_WAVES_PER_EU) void k_fast_x(XArgs a, 
 }
 
 #ifndef E3_ZF_SPLIT
-#define E3_ZF_SPLIT 0   // 1: one block per (component, pencil group) instead of all components per block
+#define E3_ZF_SPLIT 0   // 1: one block per (component, pencil group) in single precision too
 #endif
+
+// forward z blocks per (component, pencil group): FP64 (measured faster there), or all with E3_ZF_SPLIT
+template <class K>
+constexpr bool zf_split() {
+  return E3_ZF_SPLIT || sizeof(typename K::Real) == 8;
+}
 
 template <class K>
 __global__ __launch_bounds__(K::NT) void k_fast_zf(ZArgs a, long long nblk) {
   if (a.skip && *a.skip) return;
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
-#if E3_ZF_SPLIT
-  const long long bid = blockIdx.x / a.C;
-  const int c = (int)(blockIdx.x - bid * a.C);
-  const int b = (int)(bid / nblk);
-  K k(a, bid - (long long)b * nblk, b, lds);
-  k.stage_a(c, threadIdx.x);
-  __syncthreads();
-  k.stage_b(c, threadIdx.x);
-#else
-  const long long bid = blockIdx.x;
-  const int b = (int)(bid / nblk);
-  K k(a, bid - (long long)b * nblk, b, lds);
-  for (int c = 0; c < a.C; ++c) {
+  if constexpr (zf_split<K>()) {
+    const long long bid = blockIdx.x / a.C;
+    const int c = (int)(blockIdx.x - bid * a.C);
+    const int b = (int)(bid / nblk);
+    K k(a, bid - (long long)b * nblk, b, lds);
     k.stage_a(c, threadIdx.x);
     __syncthreads();
     k.stage_b(c, threadIdx.x);
-    if (c + 1 < a.C) __syncthreads();
+  } else {
+    const long long bid = blockIdx.x;
+    const int b = (int)(bid / nblk);
+    K k(a, bid - (long long)b * nblk, b, lds);
+    for (int c = 0; c < a.C; ++c) {
+      k.stage_a(c, threadIdx.x);
+      __syncthreads();
+      k.stage_b(c, threadIdx.x);
+      if (c + 1 < a.C) __syncthreads();
+    }
   }
-#endif
 }
 
 template <class K, int C>
@@ -117,7 +123,7 @@ struct HipFastRun {
   template <class K>
   void zf(const ZArgs& a) {
     const long long nblk = (a.P + K::P - 1) / K::P;
-    const long long split = E3_ZF_SPLIT ? a.C : 1;
+    const long long split = zf_split<K>() ? a.C : 1;
     hipLaunchKernelGGL(k_fast_zf<K>, dim3((unsigned)(nblk * a.batch * split)), dim3(K::NT), 0, st, a, nblk);
     err = hipGetLastError();
   }
