@@ -51,9 +51,15 @@ def make_problem(em, be, N, k0):
 
 
 def _true_residual(op64, u, f):
-    import torch
-    r = op64.matvec(u) - f
-    return float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(f))
+    from em3d_amd.judge import true_relative_residual
+    return true_relative_residual(op64, u, f)
+
+
+def _judged(em, solver, op, f):
+    """solve() between device synchronisations, then the reference's judge:
+    FP64 true residual and qualification (converged, finite, <= tol (1+1e-10))."""
+    j = em.judge_solve(solver, op, f)
+    return j.result, j.elapsed_seconds, j
 
 
 def _timed(fn):
@@ -103,30 +109,32 @@ def solve_section(em, N, k0s, learned=True):
         prob, _ = make_problem(em, be64, N, k0)
         op = em.Operator(prob)
         f = prob.wave
+        op.matvec(f)                                    # first use: plan, workspace, kernel load
         _, t_mv = _timed(lambda: [op.matvec(f) for _ in range(10)])
         _, t_setup = _timed(op.single_precision_companion)
         op.single_precision_companion().matvec(f.to(torch.complex64))  # warm the complex64 plan
-        plain, t_plain = _timed(lambda: em.BiCGStab(cfg).solve(op, f))
-        mixed, t_mixed = _timed(lambda: em.MixedBiCGStab(cfg).solve(op, f))
+        plain, t_plain, jp = _judged(em, em.BiCGStab(cfg), op, f)
+        mixed, t_mixed, jm = _judged(em, em.MixedBiCGStab(cfg), op, f)
         rec = {
             "fp64_matvec_ms": round(t_mv / 10 * 1e3, 4),
             "fp64_bicgstab": {"seconds": round(t_plain, 4), "iterations": plain.iterations,
                               "matvecs": plain.matvec_count, "status": plain.status,
-                              "true_residual": _true_residual(op, plain.u, f)},
+                              "true_residual": jp.true_residual, "qualified": jp.qualified},
             "mixed_bicgstab": {"seconds": round(t_mixed, 4), "iterations": mixed.iterations,
                                "fp32_matvecs": mixed.extras["fp32_matvecs"],
                                "fp64_matvecs": mixed.extras["fp64_matvecs"], "outer": mixed.extras["outer"],
-                               "status": mixed.status, "true_residual": _true_residual(op, mixed.u, f),
-                               "setup_seconds": round(t_setup, 4)},
+                               "status": mixed.status, "true_residual": jm.true_residual,
+                               "qualified": jm.qualified, "setup_seconds": round(t_setup, 4)},
             "solution_rel_diff": float(torch.linalg.vector_norm(mixed.u - plain.u) / torch.linalg.vector_norm(plain.u)),
             "speedup_mixed_vs_fp64": round(t_plain / t_mixed, 3),
         }
         if fno is not None:
-            lm, t_lm = _timed(lambda: em.MixedBiCGStab(cfg, preconditioner=fno).solve(op, f))
+            lm, t_lm, jl = _judged(em, em.MixedBiCGStab(cfg, preconditioner=fno), op, f)
             rec["learned_pc_mixed_bicgstab"] = {
                 "seconds": round(t_lm, 4), "iterations": lm.iterations, "fp32_matvecs": lm.extras["fp32_matvecs"],
                 "fp64_matvecs": lm.extras["fp64_matvecs"], "status": lm.status,
-                "inner_status": lm.extras["inner_status"], "true_residual": _true_residual(op, lm.u, f)}
+                "inner_status": lm.extras["inner_status"], "true_residual": jl.true_residual,
+                "qualified": jl.qualified}
             rec["speedup_learned_pc_mixed_vs_fp64"] = round(t_plain / t_lm, 3)
             log(f"[bench] k0={k0}: learned-PC mixed {t_lm:.3f} s ({lm.iterations} it, {lm.status}, "
                 f"inner {lm.extras['inner_status']})")
@@ -139,6 +147,7 @@ def solve_section(em, N, k0s, learned=True):
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "em_matvec_128_c64_hbm_traffic.txt")
+TRACE_SUMMARY = os.path.join(ROOT, "profiles", "em_matvec_128_c64_kernel_trace.txt")
 
 
 def measured_traffic(kernel, N, n_chi, es):
@@ -267,6 +276,48 @@ def acoustic_section(em, sizes=(128, 256), k0=10.0, reps=20):
     return out
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def _blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return [{"api": i.get("internal_api"), "threads": i.get("num_threads")} for i in threadpool_info()]
+    except Exception:
+        return None
+
+
+def cpu_solve_row(em):
+    """Time to rtol 1e-8 of the README's config-1 solve (16^3 sphere, eps_r 2,
+    k0 4, BiCGStab) by the oracle on the host (the reference's NumPy path,
+    restated), beside the same solve on the GPU (device BiCGStab, FP64)."""
+    from oracle import inputs, solvers
+    N, L, c = (16, 16, 16), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    mask = inputs.ellipsoid_center_mask(N, L, c, ell_center=c, radii=(0.25,) * 3)
+    chi = inputs.contrast_from_weights(mask.astype(float), background_eps_r=1.0, feature_eps_r=2.0)
+    wave = inputs.flat_wave_vec(N, L, c, k=4.0, orient=(0, 0, 1), amplitude=(1, 0, 0))
+    t0 = time.perf_counter()
+    ref = solvers.bicgstab(solvers.EMOperator(N, L, 4.0, chi), wave, solvers.Config(max_iter=200, rtol=1e-8))
+    t_cpu = time.perf_counter() - t0
+    be = em.Backend.hip(em.Precision.DOUBLE)
+    grid = em.Grid(N=N, L=L, center=c, backend=be)
+    prob = em.Problem(grid=grid, eps_tensor=be.array(chi), wave=be.array(wave), k0=4.0, volume=1.0)
+    op = em.Operator(prob)
+    em.BiCGStab(em.SolverConfig(max_iter=200, rtol=1e-8)).solve(op, prob.wave)     # first use
+    res, t_gpu, j = _judged(em, em.BiCGStab(em.SolverConfig(max_iter=200, rtol=1e-8)), op, prob.wave)
+    return {"workload": "config-1 sphere 16^3, eps_r 2, k0 4, BiCGStab to rtol 1e-8 (complex128)",
+            "cpu_seconds": round(t_cpu, 4), "cpu_iterations": ref.iterations, "cpu_status": ref.status,
+            "gpu_seconds": round(t_gpu, 5), "gpu_iterations": res.iterations, "gpu_qualified": j.qualified,
+            "speedup_gpu_vs_cpu": round(t_cpu / t_gpu, 1)}
+
+
 def cpu_baseline(N, reps, k0):
     """Oracle (NumPy, complex64 as the reference's SINGLE path) matvec at the
     same size; the spectrum is a random tensor of the reference's dense
@@ -288,9 +339,9 @@ def cpu_baseline(N, reps, k0):
         t.append(time.perf_counter() - t0)
     med = float(np.median(t))
     return {"value": 1.0 / med, "unit": "matvecs/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} oracle EM matvecs at {N}^3 complex64 after 1 warm-up (median {med:.2f} s); "
-                      f"numpy.fft is single-threaded; host {platform.processor() or platform.machine()}, "
-                      f"os.cpu_count()={os.cpu_count()}"}
+            "sample": f"{reps} oracle EM matvecs at {N}^3 complex64 after 1 warm-up (median {med:.2f} s, "
+                      f"all {', '.join(f'{x:.2f}' for x in t)} s); numpy.fft is single-threaded",
+            "cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count(), "blas_threads": _blas_threads()}
 
 
 def farfield_case(args):
@@ -537,8 +588,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, numpy)")
-        cpu = cpu_baseline(N, 2, k0)
+        cpu = cpu_baseline(N, 3, k0)
         log(f"[bench] CPU {cpu['value']:.4f} matvecs/s")
+        cpu["solve"] = cpu_solve_row(em)
+        log(f"[bench] CPU config-1 solve {cpu['solve']}")
 
     out = {
         "metric": METRIC,
@@ -568,6 +621,10 @@ def main():
                      # the committed PMC summary was measured with the default x-pass variant
                      "traffic": measured_traffic(PASS_NAMES[dom], N, n_chi, es)
                      if tuned is None or tuned["variants"][1] == 0 else None,
+                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT),
+                     "time_source": "hipEvents recorded before and after each launch on the launch stream",
+                     "trace_summary": os.path.relpath(TRACE_SUMMARY, ROOT),
+                     "launch_ms": round(per_launch[dom], 5),
                      "algorithmic_bytes": pb[dom]},
         "solve": solve,
         "batch": batch,

@@ -6,7 +6,7 @@ from __future__ import annotations
 import numpy as np
 
 from ..farfield import _integral, packed_contrast
-from ..operator import CHI_ISO
+from ..operator import CHI_ISO, eta_contrast
 
 
 def _directions_array(directions) -> np.ndarray:
@@ -38,7 +38,7 @@ def farfield_amplitude(u, problem, directions, *, batch_size: int = 64) -> np.nd
     shape = getattr(u, "shape", None)
     if shape is not None and tuple(shape) != tuple(grid.N):
         raise ValueError(f"u.shape {tuple(shape)} != expected {tuple(grid.N)}")
-    chi = packed_contrast(problem.eta, lambda eta: (eta - 1.0).contiguous())   # problem.chi, packed once
+    _, chi = packed_contrast(problem.eta, eta_contrast)   # problem.chi, packed once
     F = _integral(u, grid, problem.k0, CHI_ISO, chi, 1, dirs)[0, :, 0]
     coeff = float(problem.k0) ** 2 / (4.0 * np.pi)     # dv is applied by the integral
     return coeff * F.cpu().numpy()

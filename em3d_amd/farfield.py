@@ -21,14 +21,15 @@ _SELF = object()   # stands for the key tensor inside a cached value
 def packed_contrast(t, make):
     """make(t), kept while t is alive and unchanged: the same tensor object
     with the same version counter (torch bumps it on every in-place change).
-    The entry is dropped when t is freed.  Repeated far-field calls on one
-    problem (a sweep of planes, of fields) then pack the contrast once.  A
-    value that is t itself, or a tuple holding it (a full 3 x 3 contrast
-    packs to itself), is stored without that reference, so the cache never
-    keeps t alive."""
+    Entries are keyed by (t, make), so ``make`` must be a module-level
+    function (one cache line per packing rule).  The entry is dropped when t
+    is freed.  Repeated far-field calls and operator applications on one
+    problem then pack the contrast once.  A value that is t itself, or a
+    tuple holding it (a full 3 x 3 contrast packs to itself), is stored
+    without that reference, so the cache never keeps t alive."""
     if not hasattr(t, "_version"):
         return make(t)
-    key = id(t)
+    key = (id(t), make)
     hit = _PACKED.get(key)
     if hit is not None and hit[0]() is t and hit[1] == t._version:
         v = hit[2]

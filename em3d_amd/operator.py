@@ -211,8 +211,17 @@ class PreparedEMKernel:
 
     @property
     def persistent_nbytes(self) -> int:
+        """Bytes this object keeps in device memory: the octant spectrum.  The
+        reference's figure is the dense ``kernel_hat`` (plus its explicit
+        adjoint) it stores (operator.py:230-235); that size is
+        ``dense_kernel_hat_nbytes`` here, materialised only on request."""
         t = self.khat_octant
         return int(t.numel() * t.element_size())
+
+    @property
+    def dense_kernel_hat_nbytes(self) -> int:
+        """Size of the (3, 3, 2Nx, 2Ny, 2Nz) tensor ``kernel_hat`` returns."""
+        return 9 * 8 * prod(self.grid_shape) * int(self.khat_octant.element_size())
 
     @property
     def kernel_hat(self):
@@ -246,17 +255,24 @@ def prep_coeffs_em3d(grid: Grid, *, k: float, volume: float | None = None):
 
 
 def pack_contrast(chi):
-    """Classify a (3, 3, Nx, Ny, Nz) device contrast as iso / diag / full and pack it."""
+    """Classify a (3, 3, Nx, Ny, Nz) device contrast as iso / diag / full and
+    pack it.  The off-diagonal test reads the six slices in place (no copy of
+    the tensor); an isotropic contrast packs to a fresh (Nx, Ny, Nz) tensor,
+    so nothing keeps the 3 x 3 input alive."""
     import torch
-    off = chi.clone()
     for a in range(3):
-        off[a, a] = 0
-    if bool(torch.count_nonzero(off).item()):
-        return CHI_FULL, chi.contiguous()
-    d = torch.stack([chi[a, a] for a in range(3)])
-    if bool(torch.equal(d[0], d[1])) and bool(torch.equal(d[0], d[2])):
-        return CHI_ISO, d[0].contiguous()
-    return CHI_DIAG, d.contiguous()
+        for b in range(3):
+            if a != b and bool(torch.count_nonzero(chi[a, b]).item()):
+                return CHI_FULL, chi.contiguous()
+    if bool(torch.equal(chi[0, 0], chi[1, 1])) and bool(torch.equal(chi[0, 0], chi[2, 2])):
+        return CHI_ISO, chi[0, 0].clone(memory_format=torch.contiguous_format)
+    return CHI_DIAG, torch.stack([chi[a, a] for a in range(3)])
+
+
+def eta_contrast(eta):
+    """Acoustic contrast ``chi = eta - 1`` (acoustics/problem.py:73-79) packed
+    as an isotropic contrast."""
+    return CHI_ISO, (eta - 1.0).contiguous()
 
 
 class _FieldApplier:
@@ -270,6 +286,23 @@ class _FieldApplier:
         self._k0 = None if k0 is None else float(k0)
         self._work: dict[int, object] = {}
         self._companion = None
+        self._chi_src = None
+        self._chi_version = None
+
+    def _refresh_contrast(self, source, make):
+        """(kind, packed contrast) of ``source``, re-packed when ``source`` is
+        another tensor or its version counter moved (torch bumps it on every
+        in-place change): the reference reads the problem's contrast on every
+        call (operator.py:353-357), so an in-place edit must reach the next
+        matvec, adjoint and solve.  The complex64 companion, which holds a
+        cast of the old contrast, is dropped then."""
+        from .farfield import packed_contrast
+        v = getattr(source, "_version", None)
+        if source is not self._chi_src or v != self._chi_version:
+            self._chi_kind, self._chi = packed_contrast(source, make)
+            self._chi_src, self._chi_version = source, v
+            self._companion = None
+        return self._chi_kind, self._chi
 
     def _spectrum(self):
         """(octant spectrum, chi kind, packed chi) this operator applies."""
@@ -279,12 +312,12 @@ class _FieldApplier:
         """The same operator in complex64: spectrum rebuilt in FP64 and cast,
         contrast cast from this operator's.  Built once and cached."""
         import torch
+        _, kind, chi = self._spectrum()      # drops a companion built on a contrast edited since
         if self._companion is None:
             if self._be.precision is Precision.SINGLE:
                 self._companion = self
             else:
                 from .backend import Backend
-                _, kind, chi = self._spectrum()
                 be32 = Backend.hip(Precision.SINGLE, self._be.device_index)
                 khat = build_octant_spectrum(self._N, self._L, self._k0, self._ncomp, Precision.SINGLE,
                                              self._be.device_index)
@@ -455,10 +488,11 @@ class Operator(_FieldApplier):
             prepared_kernel.validate(grid, k=problem.k0)
         self._prepared_kernel = prepared_kernel
         self._init_applier(grid.backend, grid.N, 3, grid.L, problem.k0)
-        self._chi_kind, self._chi = pack_contrast(problem.eps_tensor)
+        self._refresh_contrast(problem.eps_tensor, pack_contrast)
 
     def _spectrum(self):
-        return self._prepared_kernel.khat_octant, self._chi_kind, self._chi
+        kind, chi = self._refresh_contrast(self.problem.eps_tensor, pack_contrast)
+        return self._prepared_kernel.khat_octant, kind, chi
 
     @property
     def backend(self):
@@ -475,20 +509,19 @@ class Operator(_FieldApplier):
 
     @property
     def contrast_kind(self) -> str:
-        return {CHI_ISO: "isotropic", CHI_DIAG: "diagonal", CHI_FULL: "full"}[self._chi_kind]
+        return {CHI_ISO: "isotropic", CHI_DIAG: "diagonal", CHI_FULL: "full"}[self._spectrum()[1]]
 
     def matvec(self, field):
-        return self._apply(field, self._prepared_kernel.khat_octant, self._chi_kind, self._chi, False)
+        return self._apply(field, *self._spectrum(), False)
 
     def device_bicgstab(self, rhs, cfg, batched: bool = False, precond=None):
         """BiCGStab with all iteration scalars on the device (libem3d_hip)."""
-        return self._device_bicgstab(rhs, cfg, self._prepared_kernel.khat_octant, self._chi_kind, self._chi,
-                                     batched, precond)
+        return self._device_bicgstab(rhs, cfg, *self._spectrum(), batched, precond)
 
     def rmatvec(self, field):
         if not self._prepared_kernel.supports_adjoint:
             raise RuntimeError("adjoint action is unavailable: build PreparedEMKernel with include_adjoint=True")
-        return self._apply(field, self._prepared_kernel.khat_octant, self._chi_kind, self._chi, True)
+        return self._apply(field, *self._spectrum(), True)
 
     def matvec_flat(self, vector):
         return flatten_field(self.matvec(unflatten_field(vector, self._N)))

@@ -81,11 +81,13 @@ def _prec(x) -> int:
     raise TypeError(f"HIP reductions take complex64/complex128 tensors, got {x.dtype}")
 
 
-_red_work: dict[int, torch.Tensor] = {}
+_red_work: dict[tuple[int, int], torch.Tensor] = {}
 
 
 def _work(dev) -> torch.Tensor:
-    key = dev.index if dev.index is not None else 0
+    """Partial-sum scratch of em3d_dotc, one per (device, stream): reductions
+    issued on two streams at once never share partials."""
+    key = (dev.index if dev.index is not None else 0, _lib.stream_handle(dev))
     w = _red_work.get(key)
     if w is None:
         w = torch.empty(int(_lib.lib().em3d_reduce_work_bytes()), dtype=torch.uint8, device=dev)

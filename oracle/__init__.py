@@ -28,6 +28,9 @@ restatement) is listed per function in ``PINNING`` below and exercised by
 * TwoStep               -- closed-form two-update recurrence on a 2x2 matrix
                            (the reference's own known-answer test); manufactured
                            solutions
+* input builders        -- exact rational axes, plane-wave values at cells with
+                           known phases, mask counts by integer arithmetic on
+                           doubled cell indices (surface ties included)
 """
 
 PINNING = {
@@ -39,4 +42,14 @@ PINNING = {
     "solvers.bicgstab": "scipy.sparse.linalg.bicgstab iterates; manufactured solution",
     "solvers.sim": "diagonal closed form; manufactured solution",
     "solvers.twostep": "2x2 two-update closed form; manufactured solution",
+    "solvers.true_relative_residual": "dense operator residual on small grids (tests/test_oracle_solvers.py)",
+    "inputs.axis": "exact rational cell centres (fractions) to 4 ulp; span; antisymmetry",
+    "inputs.flat_wave_vec": "values at cells whose phase is a multiple of pi/2; modulus; phase step; sign",
+    "inputs.plane_wave_scalar": "values at cells whose phase is a multiple of pi/2",
+    "inputs.ellipsoid_center_mask": "integer counts on doubled cell indices; 33 cells for exact surface ties",
+    "inputs.box_center_mask": "integer counts per axis",
+    "inputs.contrast_from_weights": "eps_f - I / eps_bg - I at chosen cells, untransposed",
+    "inputs.ellipsis_refraction": "integer counts; entries of a diagonal lossy tensor",
+    "inputs.cylinder_refraction": "integer disc count times the axis length",
+    "inputs.eta_ellipsoid": "integer counts inside and outside",
 }

@@ -328,54 +328,72 @@ def test_batched_bicgstab_matches_single_solves(em):
 
 
 @pytest.mark.parametrize("prec,tol", [("double", 1e-13), ("single", 2e-6)])
-def test_plane_waves_match_flat_wave_vec(em, prec, tol):
+def test_plane_waves_match_oracle(em, prec, tol):
+    """Device plane waves (em3d_plane_waves) against the oracle's flat_wave_vec
+    (src/em3d/wave.py:9-37) and against the product's host flat_wave_vec.  The
+    device phase is FP64; the FP32 oracle forms it in float32, hence 2e-6."""
     from em3d_amd.sources import incident_set, plane_waves
+    from oracle import inputs
     be = _be(em, prec)
-    grid = em.Grid(N=(12, 10, 8), L=(1.0, 0.8, 1.3), center=(0.1, -0.2, 0.05), backend=be)
+    N, L, c = (12, 10, 8), (1.0, 0.8, 1.3), (0.1, -0.2, 0.05)
+    grid = em.Grid(N=N, L=L, center=c, backend=be)
     o, a = incident_set(3, seed=7)
     a = a * (0.5 - 0.25j)
     dev = plane_waves(grid, k=9.0, orients=o, amplitudes=a, phi0=0.3, sign=-1)
     for b in range(o.shape[0]):
-        ref = em.flat_wave_vec(grid, k=9.0, orient=tuple(o[b]), amplitude=tuple(a[b]), phi0=0.3, sign=-1)
-        assert _rel(_host(dev[b]), _host(ref)) < tol
+        ref = inputs.flat_wave_vec(N, L, c, k=9.0, orient=tuple(o[b]), amplitude=tuple(a[b]), phi0=0.3, sign=-1,
+                                   precision=prec)
+        assert _rel(_host(dev[b]), ref.astype(np.complex128)) < tol
+        host = em.flat_wave_vec(grid, k=9.0, orient=tuple(o[b]), amplitude=tuple(a[b]), phi0=0.3, sign=-1)
+        assert _rel(_host(dev[b]), _host(host)) < tol
 
 
 @pytest.mark.parametrize("prec", ["double", "single"])
-def test_device_materials_match_host(em, prec):
-    """Device ellipsoid contrast and eta inclusions equal the host sampling
-    bit for bit, boundary cells included (a sphere through cell centres)."""
+def test_device_materials_match_oracle(em, prec):
+    """Device ellipsoid contrast (em3d_ellipsoid_contrast) and inclusion eta
+    (em3d_eta_inclusions) equal the oracle's cell-centre sampling bit for bit
+    (src/em3d/geometry.py:62-93, 233-252; acoustics/materials.py:62-83), cells
+    on the surface included, and agree with the product's host builders."""
     import torch
-    from em3d_amd.acoustics.materials import eta_ellipsoid, eta_inclusions, random_spheres
+    from em3d_amd.acoustics.materials import eta_inclusions, random_spheres
+    from oracle import inputs
     be = _be(em, prec)
-    grid = em.Grid(N=(16, 16, 16), L=(1, 1, 1), center=(0, 0, 0), backend=be)
+    N, L, c = (16, 16, 16), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0)
+    grid = em.Grid(N=N, L=L, center=c, backend=be)
+    cdt = np.complex128 if prec == "double" else np.complex64
     geo = em.geometry
     eps1 = np.diag([2.5 + 0.1j, 3.0 + 0.2j, 2.0])
     eps2 = np.array([[2.0, 0.3j, 0], [-0.3j, 2.0, 0], [0, 0, 1.5]])
-    # radius 0.28125 = 4.5 cells: centres exactly on the surface test the tolerance
-    e1, e2 = geo.Ellipsoid(center=(0, 0, 0), radii=(0.28125,) * 3), geo.Ellipsoid(center=(0.1, 0, 0),
-                                                                               radii=(0.2, 0.15, 0.1))
-    ref1 = geo.sample_contrast_tensor(grid, background_eps_r=1.0, feature_eps_r=eps1, geometry=e1)
-    assert torch.equal(geo.ellipsoids_contrast(grid, [(e1, eps1)]), ref1)
-    d = geo.ellipsoids_contrast(grid, [(e1, eps1)], kind="diag")
-    assert torch.equal(d, torch.stack([ref1[i, i] for i in range(3)]))
-    two = geo.ellipsoids_contrast(grid, [(e1, eps1), (e2, eps2)], background_eps_r=1.2)
-    m1 = torch.as_tensor(geo.center_mask(e1, grid), device=two.device)
-    m2 = torch.as_tensor(geo.center_mask(e2, grid), device=two.device)
+    # centre on a cell centre, radius 2 cells: the axis cells at distance 2 lie on the surface
+    c0 = float(inputs.axis(16, 1.0, 0.0, prec)[7])
+    e1 = geo.Ellipsoid(center=(c0, c0, c0), radii=(0.125,) * 3)
+    e2 = geo.Ellipsoid(center=(0.1, 0, 0), radii=(0.2, 0.15, 0.1))
+    m1 = inputs.ellipsoid_center_mask(N, L, c, ell_center=e1.center, radii=e1.radii, precision=prec)
+    m2 = inputs.ellipsoid_center_mask(N, L, c, ell_center=e2.center, radii=e2.radii, precision=prec)
+    assert int(m1.sum()) == 33
+    ref1 = inputs.contrast_from_weights(m1.astype(float), background_eps_r=1.0, feature_eps_r=eps1, precision=prec)
+    got1 = _host(geo.ellipsoids_contrast(grid, [(e1, eps1)]))
+    assert np.array_equal(got1, ref1)
+    assert np.array_equal(_host(geo.ellipsoids_contrast(grid, [(e1, eps1)], kind="diag")),
+                          np.stack([ref1[i, i] for i in range(3)]))
+    assert torch.equal(geo.ellipsoids_contrast(grid, [(e1, eps1)]),
+                       geo.sample_contrast_tensor(grid, background_eps_r=1.0, feature_eps_r=eps1, geometry=e1))
+    two = _host(geo.ellipsoids_contrast(grid, [(e1, eps1), (e2, eps2)], background_eps_r=1.2))
     for a in range(3):
-        for c in range(3):
-            bg = (1.2 if a == c else 0.0) - (1.0 if a == c else 0.0)
-            want = torch.full(grid.N, bg, dtype=two.dtype, device=two.device)
-            want = torch.where(m1, torch.tensor(eps1[a, c] - (a == c), dtype=two.dtype, device=two.device), want)
-            want = torch.where(m2, torch.tensor(eps2[a, c] - (a == c), dtype=two.dtype, device=two.device), want)
-            assert torch.equal(two[a, c], want), (a, c)
+        for b in range(3):
+            want = np.full(N, (1.2 - 1.0) if a == b else 0.0, dtype=cdt)
+            want = np.where(m1, cdt(eps1[a, b] - (a == b)), want)
+            want = np.where(m2, cdt(eps2[a, b] - (a == b)), want)
+            assert np.array_equal(two[a, b], want), (a, b)
     spheres = random_spheres(8, seed=3, radius=(0.1, 0.2))
-    spheres.append(((0.0, 0.0, 0.0), (0.28125,) * 3, 2.0 + 0.1j))
-    eta = eta_inclusions(grid, spheres)
-    want = torch.ones(grid.N, dtype=eta.dtype, device=eta.device)
-    for c0, r, v in spheres:
-        m = eta_ellipsoid(grid, center=c0, radii=r, eta_inside=1.0, eta_outside=0.0) != 0
-        want = torch.where(m, torch.tensor(v, dtype=eta.dtype, device=eta.device), want)
-    assert torch.equal(eta, want)
+    spheres.append(((c0, c0, c0), (0.125,) * 3, 2.0 + 0.1j))
+    eta = _host(eta_inclusions(grid, spheres))
+    want = np.ones(N, dtype=cdt)
+    for cc, r, v in spheres:
+        m = inputs.eta_ellipsoid(N, L, c, ell_center=cc, radii=r, eta_inside=1.0, eta_outside=0.0,
+                                 precision=prec) != 0
+        want = np.where(m, cdt(v), want)
+    assert np.array_equal(eta, want)
 
 
 def test_autotuner_keeps_results(em):

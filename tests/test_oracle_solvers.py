@@ -108,3 +108,15 @@ def test_action_budgets(name):
     r = getattr(solvers, name)(op, rhs, cfg)
     assert r.matvec_count + r.rmatvec_count <= 2
     assert r.status in {"converged", "max_operator_actions"}
+
+
+def test_true_relative_residual_closed_form():
+    """A = 2 I, u = f / 2 + d: |A u - f| / |f| = 2 |d| / |f|; f = 0 gives |A u|."""
+    from oracle import solvers
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal(10) + 1j * rng.standard_normal(10)
+    d = 1e-3 * (rng.standard_normal(10) + 0j)
+    op = solvers.MatrixOperator(2.0 * np.eye(10))
+    r = solvers.true_relative_residual(op, f / 2 + d, f)
+    assert abs(r - 2 * np.linalg.norm(d) / np.linalg.norm(f)) < 1e-15
+    assert solvers.true_relative_residual(op, d, np.zeros(10)) == 2 * np.linalg.norm(d)
