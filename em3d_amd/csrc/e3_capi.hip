@@ -327,18 +327,23 @@ int em3d_plan_autotune(em3d_plan_t plan, const void* khat, const void* chi, int 
     *ms = (double)t / reps;
     return 0;
   };
-  // coordinate search: the x pass first (it dominates), then the strided and z passes
+  // coordinate search: the x pass first (it dominates), then the strided and z
+  // passes.  The plan's current variant is timed first and kept unless another
+  // is faster by more than 1 %, so run-to-run noise does not flip the choice.
   const int order[3] = {PASS_X, PASS_S, PASS_Z};
   for (int oi = 0; oi < 3 && !rc; ++oi) {
     const int k = order[oi];
     const int n = em3d_plan_variant_count(plan, k);
-    int best_v = plan->p.variant[k];
+    const int incumbent = plan->p.variant[k];
+    int best_v = incumbent;
     double best = -1.0;
-    for (int v = -1; v < n && !rc; ++v) {
+    for (int i = -1; i <= n && !rc; ++i) {
+      const int v = i < 0 ? incumbent : i - 1;     // incumbent, then -1 .. n-1
+      if (i >= 0 && v == incumbent) continue;
       plan->p.variant[k] = v;
       double ms = 0.0;
       rc = measure(&ms);
-      if (!rc && (best < 0.0 || ms < best)) {
+      if (!rc && (best < 0.0 || ms < 0.99 * best)) {
         best = ms;
         best_v = v;
       }
