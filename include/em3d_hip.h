@@ -78,6 +78,20 @@ int em3d_matvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kin
 int em3d_rmatvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* v, void* y,
                  void* work, hipStream_t stream);
 
+/* em3d_matvec without a caller workspace: the FFT workspace is a fixed
+ * file-scope device array of EM3D_NOALLOC_SCRATCH_BYTES per GPU (held until
+ * the process ends), so the call allocates nothing.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Fails with a message when the plan's workspace (em3d_plan_sizes
+ * work_bytes) exceeds the array: 64 MiB holds EM grids up to 64^3 complex64
+ * (37.7 MB) and acoustic grids up to 64^3 complex128.  The plan must belong
+ * to the current device.  Calls on one GPU share the array: calls that
+ * overlap on that GPU (two streams, two threads) race. */
+#define EM3D_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int em3d_matvec_noalloc(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
+                        hipStream_t stream);
+
 /* Per-launch timing of em3d_matvec/em3d_rmatvec on this plan, the measured
  * counterpart of the reference's matvec timer (benchmark_matvec,
  * experiments/scans.py:275-300).  While enabled, each apply call records a
