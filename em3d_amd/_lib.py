@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(HERE, "libem3d_hip.so")
 
 class BiCGStabParams(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("rtol", C.c_double), ("atol", C.c_double),
-                ("divergence_guard", C.c_double), ("max_operator_actions", C.c_longlong), ("chunk", C.c_int)]
+                ("divergence_guard", C.c_double), ("max_operator_actions", C.c_longlong), ("chunk", C.c_int),
+                ("fused", C.c_int)]
 
 
 class SolverResultC(C.Structure):

@@ -25,6 +25,46 @@ enum Prologue { PRO_COPY = 0, PRO_CHI = 1 };
 enum Epilogue { EPI_STORE = 0, EPI_SUB = 1, EPI_SUB_CHIH = 2 };
 
 // ---------------------------------------------------------------------------
+// Krylov steps of the device BiCGStab fused into the z passes of its two
+// matvecs (solvers/bicgstab.py:96-116, 145-175 of the reference):
+//   KRY_P    forward z prologue of v = A p:  p = r + beta (p - omega v)
+//            (p = r on the first iteration), written back, then chi p;
+//   KRY_S    forward z prologue of t = A s:  s = r - alpha v, written, then
+//            chi s, with the partial sums of |s|^2;
+//   KRY_DOT  inverse z epilogue of v = A p:  partial sums of (r_hat, v);
+//   KRY_DOT2 inverse z epilogue of t = A s:  partial sums of (t, t), (t, s).
+// The coefficients come per right-hand side from the solver's decide kernels
+// (KryCoef); a right-hand side that is done keeps its fields.  Partial sums
+// go to part[(b * part_stride + block) * 4 + i], block counted within the
+// right-hand side, and the number of blocks per right-hand side to *count.
+// The prologues are elementwise per component, so they need a contrast that
+// does not mix components (none, iso, diag).
+// ---------------------------------------------------------------------------
+enum KryOp { KRY_NONE = 0, KRY_P = 1, KRY_S = 2, KRY_DOT = 3, KRY_DOT2 = 4 };
+
+struct KryCoef {
+  double beta[2], omega[2], alpha[2];
+  int first, done;
+};
+
+struct ZKry {
+  int op;                  // KryOp
+  const void* r;           // KRY_P, KRY_S
+  const void* v;           // KRY_P, KRY_S
+  void* w;                 // KRY_P, KRY_S: the updated p or s (the pass input)
+  const void* rh;          // KRY_DOT
+  const KryCoef* coef;     // per right-hand side
+  double* part;
+  long long part_stride;   // partial records per right-hand side
+  int* count;              // blocks per right-hand side (written by block 0)
+};
+
+// per-thread partial sums of a fused reduction
+struct KAcc {
+  double v[4];
+};
+
+// ---------------------------------------------------------------------------
 // Stockham stage over ncol pencils held in LDS (pencil p at base + p*ps).
 // Butterfly j of radix R with Ns = product of the radices already applied:
 //   v[m] = in[j + m L/R] * W_{Ns R}^{(j mod Ns) m};  v = DFT_R(v);
@@ -169,7 +209,51 @@ struct ZArgs {
   int variant;         // fast-kernel variant chosen by the plan (-1: the generic kernels)
   int nt;              // fast kernels: rows read and written non-temporally
   const int* skip;             // device flag: when non-zero the whole pass is a no-op
+  ZKry k;                      // fused Krylov step (k.op == KRY_NONE: none)
 };
+
+// The Krylov prologue value of component c at element o (offset within the
+// right-hand side b), before the contrast: KRY_P p, KRY_S s (and |s|^2 into
+// acc).  Same arithmetic, in T, as the stand-alone vector kernels.
+template <typename T>
+E3_HD cpx<T> kry_update(const ZArgs& a, int b, size_t o, KAcc& acc) {
+  const KryCoef& k = a.k.coef[b];
+  const size_t bo = (size_t)b * a.in_bs + o;
+  const cpx<T> r = ((const cpx<T>*)a.k.r)[bo];
+  const cpx<T> v = ((const cpx<T>*)a.k.v)[bo];
+  cpx<T>* w = (cpx<T>*)a.k.w + bo;
+  cpx<T> x;
+  if (a.k.op == KRY_P) {
+    if (k.first) {
+      x = r;
+    } else {
+      const cpx<T> om = mk((T)k.omega[0], (T)k.omega[1]), be = mk((T)k.beta[0], (T)k.beta[1]);
+      const cpx<T> d = *w - om * v;
+      x = r + be * d;
+    }
+  } else {
+    const cpx<T> al = mk((T)k.alpha[0], (T)k.alpha[1]);
+    x = r - al * v;
+    acc.v[0] += (double)x.re * x.re + (double)x.im * x.im;
+  }
+  *w = x;
+  return x;
+}
+
+// Krylov epilogue sums of one stored output y (src: the matvec input at the
+// same element, b-relative offset o)
+template <typename T>
+E3_HD void kry_epilogue(const ZArgs& a, int b, size_t o, cpx<T> y, cpx<T> src, KAcc& acc) {
+  if (a.k.op == KRY_DOT) {
+    const cpx<T> h = ((const cpx<T>*)a.k.rh)[(size_t)b * a.out_bs + o];
+    acc.v[0] += (double)h.re * y.re + (double)h.im * y.im;
+    acc.v[1] += (double)h.re * y.im - (double)h.im * y.re;
+  } else if (a.k.op == KRY_DOT2) {
+    acc.v[0] += (double)y.re * y.re + (double)y.im * y.im;
+    acc.v[2] += (double)y.re * src.re + (double)y.im * src.im;
+    acc.v[3] += (double)y.re * src.im - (double)y.im * src.re;
+  }
+}
 
 template <typename T>
 struct ZPass {
@@ -181,9 +265,10 @@ struct ZPass {
   E3_HD ZPass(const ZArgs& args, long long blk, int bidx, cpx<T>* lds)
       : a(args), p0(blk * args.T), b(bidx), b0(lds), b1(lds + (size_t)args.C * args.T * args.ps) {}
 
-  E3_HD void load(int tid, int nt) const {
+  E3_HD void load(int tid, int nt, KAcc& acc) const {
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs;
     const int L = a.plan.L;
+    const bool kry = (a.k.op == KRY_P || a.k.op == KRY_S) && !a.k.coef[b].done;
     for (int e = tid; e < a.T * L; e += nt) {
       const int t = e / L, z = e - t * L;
       const long long p = p0 + t;
@@ -191,7 +276,10 @@ struct ZPass {
       if (p < a.P && z < a.n_in) {
         cpx<T> v[6];
         const int zi = (a.perm && a.inv) ? mirror_position(z, a.n_in / 2, a.n_in) : z;
-        for (int c = 0; c < a.C; ++c) v[c] = in[c * a.in_cs + p * a.in_ps + zi];
+        for (int c = 0; c < a.C; ++c) {
+          const size_t o = c * a.in_cs + p * a.in_ps + zi;
+          v[c] = kry ? kry_update<T>(a, b, o, acc) : in[o];
+        }
         if (a.pro == PRO_CHI)
           chi_apply<T>((const cpx<T>*)a.chi, a.chi_kind, a.ncell, p * a.n_in + z, a.C, v, w, false);
         else
@@ -205,7 +293,7 @@ struct ZPass {
   E3_HD void stage(int s, int tid, int nt) const {
     fft_stage<T>(a.plan, s, a.inv != 0, b0, b1, a.C * a.T, a.ps, (const cpx<T>*)a.tw, tid, nt);
   }
-  E3_HD void store(int tid, int nt) const {
+  E3_HD void store(int tid, int nt, KAcc& acc) const {
     const cpx<T>* res = fft_result<T>(a.plan, b0, b1);
     cpx<T>* out = (cpx<T>*)a.out + (size_t)b * a.out_bs;
     const cpx<T>* src = (const cpx<T>*)a.u + (size_t)b * a.out_bs;
@@ -223,7 +311,13 @@ struct ZPass {
       const int zo = (a.perm && !a.inv) ? mirror_position(z, a.n_out / 2, a.n_out) : z;
       for (int c = 0; c < a.C; ++c) {
         const size_t o = c * a.out_cs + p * a.out_ps + zo;
-        out[o] = (a.epi == EPI_STORE) ? v[c] : src[o] - v[c];
+        if (a.epi == EPI_STORE) {
+          out[o] = v[c];
+        } else {
+          const cpx<T> x = src[o], y = x - v[c];
+          out[o] = y;
+          if (a.k.op == KRY_DOT || a.k.op == KRY_DOT2) kry_epilogue<T>(a, b, o, y, x, acc);
+        }
       }
     }
   }

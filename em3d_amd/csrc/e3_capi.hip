@@ -41,13 +41,15 @@ __global__ __launch_bounds__(256) void k_zpass(ZArgs a, long long nblk) {
   const long long bid = blockIdx.x;
   const int b = (int)(bid / nblk);
   ZPass<T> z(a, bid - (long long)b * nblk, b, lds);
-  z.load(threadIdx.x, blockDim.x);
+  KAcc acc{};
+  z.load(threadIdx.x, blockDim.x, acc);
   __syncthreads();
   for (int s = 0; s < a.plan.nst; ++s) {
     z.stage(s, threadIdx.x, blockDim.x);
     __syncthreads();
   }
-  z.store(threadIdx.x, blockDim.x);
+  z.store(threadIdx.x, blockDim.x, acc);
+  if (a.k.op == KRY_S || a.k.op == KRY_DOT || a.k.op == KRY_DOT2) kry_block_store(a, acc);
 }
 
 template <typename T>
@@ -99,10 +101,18 @@ __global__ __launch_bounds__(256) void k_koct(KBArgs a, long long n) {
   if (i < n) octant_point<T>(a, i);
 }
 
+// Lets a kernel take up to kLdsMax bytes of LDS, its static LDS included.  A
+// refused attribute is not an error of the caller's later calls: the pending
+// error is cleared here.
 template <typename K>
 void allow_big_lds(K kernel) {
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsMax);
+  const void* f = reinterpret_cast<const void*>(kernel);
+  hipFuncAttributes fa;
+  size_t stat = 0;
+  if (hipFuncGetAttributes(&fa, f) == hipSuccess) stat = fa.sharedSizeBytes;
+  else (void)hipGetLastError();
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(kLdsMax - stat)) != hipSuccess)
+    (void)hipGetLastError();
 }
 
 struct HipLauncher {
@@ -180,7 +190,8 @@ void set_lds_limits_once() {
 }  // namespace
 
 int e3::apply_plan(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
-                   void* work, hipStream_t stream, bool adjoint, const char* name, const int* skip) {
+                   void* work, hipStream_t stream, bool adjoint, const char* name, const int* skip,
+                   const ZKry* k1, const ZKry* k5, const int* skip1, unsigned passes) {
   if (!plan || !khat || !u || !y || !work) return fail(std::string(name) + ": null argument");
   if (chi_kind < CHI_NONE || chi_kind > CHI_FULL) return fail(std::string(name) + ": bad chi_kind");
   if (chi_kind != CHI_NONE && !chi) return fail(std::string(name) + ": null chi");
@@ -188,7 +199,9 @@ int e3::apply_plan(em3d_plan_t plan, const void* khat, const void* chi, int chi_
   if (u == y) return fail(std::string(name) + ": output must not alias the input");
   HipLauncher go{stream};
   if (plan->timing) go.ev = plan->ev;
-  run_apply(go, plan->p, khat, chi, chi_kind, u, y, work, adjoint, skip);
+  if ((k1 && k1->op != KRY_NONE && (chi_kind == CHI_FULL || adjoint)) || (k5 && k5->op != KRY_NONE && adjoint))
+    return fail(std::string(name) + ": fused Krylov steps need a matvec with a non-mixing contrast");
+  run_apply(go, plan->p, khat, chi, chi_kind, u, y, work, adjoint, skip, k1, k5, skip1, passes);
   if (plan->timing && go.launches <= kMaxTimedLaunches) {
     go.note(hipEventRecord(plan->ev[go.launches], stream));
     go.note(hipEventSynchronize(plan->ev[go.launches]));

@@ -891,6 +891,81 @@ E3_HD cpx<T> contrast_row(const ZArgs& a, const cpx<T>* in, int c, long long p, 
   return acc;
 }
 
+// Fused Krylov prologue of one thread's MA live elements of a pencil row
+// (n = j + m R2): x = r + beta (p - omega v) (KRY_P; x = r on the first
+// iteration) or x = r - alpha v (KRY_S, |x|^2 into acc), stored to w, then
+// v[m] = chi[n] x (chi null: x).  __restrict__ lets every load go first.
+template <typename T, int R1, int R2, int MA>
+E3_HD void kry_row(const cpx<T>* __restrict__ r, const cpx<T>* __restrict__ vv, cpx<T>* __restrict__ w,
+                   const cpx<T>* __restrict__ chi, int op, const KryCoef k, int j, cpx<T>* out, KAcc& acc) {
+  cpx<T> rr[MA], vx[MA], pw[MA], cx[MA];
+  const bool upd = op == KRY_P && !k.first;
+#pragma unroll
+  for (int m = 0; m < MA; ++m) {
+    const int n = j + m * R2;
+    rr[m] = r[n];
+    vx[m] = (op == KRY_S || upd) ? vv[n] : czero<T>();
+    pw[m] = upd ? w[n] : czero<T>();
+    cx[m] = chi ? chi[n] : mk((T)1, (T)0);
+  }
+  const cpx<T> om = mk((T)k.omega[0], (T)k.omega[1]), be = mk((T)k.beta[0], (T)k.beta[1]);
+  const cpx<T> al = mk((T)k.alpha[0], (T)k.alpha[1]);
+#pragma unroll
+  for (int m = 0; m < MA; ++m) {
+    cpx<T> x;
+    if (op == KRY_P) {
+      if (upd) {
+        const cpx<T> d = pw[m] - om * vx[m];
+        x = rr[m] + be * d;
+      } else {
+        x = rr[m];
+      }
+    } else {
+      x = rr[m] - al * vx[m];
+      acc.v[0] += (double)x.re * x.re + (double)x.im * x.im;
+    }
+    w[j + m * R2] = x;
+    out[m] = chi ? cx[m] * x : x;
+  }
+}
+
+// Inverse z epilogue y = u - t of one thread's NB x MH outputs of a pencil
+// row (z = j + bb R2 + m R1) with the fused Krylov sums: (r_hat, y) (KRY_DOT)
+// or (y, y) and (y, u) (KRY_DOT2).  Loads first, then stores.
+// chi (when given): y = u - conj(chi) t (the adjoint's epilogue, a contrast
+// that does not mix components); op KRY_NONE: no sums.
+template <typename T, int NB, int MH, int R1, int R2>
+E3_HD void kry_epilogue_row(cpx<T>* __restrict__ out, const cpx<T>* __restrict__ src,
+                            const cpx<T>* __restrict__ rh, const cpx<T>* __restrict__ chi, int op, int j,
+                            const cpx<T> (*t)[MH], KAcc& acc) {
+  cpx<T> x[NB][MH], h[NB][MH];
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+    for (int m = 0; m < MH; ++m) {
+      const int z = j + bb * R2 + m * R1;
+      x[bb][m] = src[z];
+      h[bb][m] = rh ? rh[z] : (chi ? chi[z] : czero<T>());
+    }
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+    for (int m = 0; m < MH; ++m) {
+      const int z = j + bb * R2 + m * R1;
+      const cpx<T> y = x[bb][m] - (chi ? cmulc(h[bb][m], t[bb][m]) : t[bb][m]);
+      out[z] = y;
+      if (op == KRY_NONE) continue;
+      if (op == KRY_DOT) {
+        acc.v[0] += (double)h[bb][m].re * y.re + (double)h[bb][m].im * y.im;
+        acc.v[1] += (double)h[bb][m].re * y.im - (double)h[bb][m].im * y.re;
+      } else {
+        acc.v[0] += (double)y.re * y.re + (double)y.im * y.im;
+        acc.v[2] += (double)y.re * x[bb][m].re + (double)y.im * x[bb][m].im;
+        acc.v[3] += (double)y.re * x[bb][m].im - (double)y.im * x[bb][m].re;
+      }
+    }
+}
+
 template <typename T, int L, int R1_, int P_, bool PIN, bool NTZ = false, bool NTU = false>
 struct FastZF {
   using Real = T;
@@ -920,6 +995,26 @@ struct FastZF {
 #pragma unroll
     for (int m = 0; m < R1; ++m) v[m] = (m < MA && ok) ? contrast_row<T, NTZ, NTU>(a, in, c, p, j + m * R2) : czero<T>();
   }
+  // stage A input with a fused Krylov prologue (KRY_P / KRY_S, iso or diag
+  // contrast): the update of each live element, written back, then chi.  The
+  // pencil rows go through __restrict__ pointers and every load of the row is
+  // issued before its first store (kry_row).
+  E3_HD void load_a_kry(int c, int tid, cpx<T>* v, KAcc& acc) const {
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);
+    E3_OPAQUE(pl);
+    const long long p = p0 + pl;
+    const KryCoef k = a.k.coef[b];
+#pragma unroll
+    for (int m = 0; m < R1; ++m) v[m] = czero<T>();
+    if (p >= a.P || k.done) return;
+    const size_t row = (size_t)b * a.in_bs + (size_t)c * a.in_cs + (size_t)p * a.in_ps;
+    const cpx<T>* chi = nullptr;
+    if (a.pro == PRO_CHI && a.chi_kind != CHI_NONE)
+      chi = (const cpx<T>*)a.chi + (a.chi_kind == CHI_ISO ? 0 : (size_t)c * a.ncell) + (size_t)p * a.n_in;
+    kry_row<T, R1, R2, MA>((const cpx<T>*)a.k.r + row, (const cpx<T>*)a.k.v + row, (cpx<T>*)a.k.w + row, chi,
+                           a.k.op, k, j, v, acc);
+  }
   E3_HD void fft_a(int tid, cpx<T>* v) const {
     int j = tid % R2, pl = tid / R2;
     E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
@@ -931,6 +1026,12 @@ struct FastZF {
   E3_HD void stage_a(int c, int tid) const {
     cpx<T> v[R1];
     load_a(c, tid, v);
+    fft_a(tid, v);
+  }
+  E3_HD void stage_a(int c, int tid, KAcc& acc) const {
+    cpx<T> v[R1];
+    if (a.k.op == KRY_P || a.k.op == KRY_S) load_a_kry(c, tid, v, acc);
+    else load_a(c, tid, v);
     fft_a(tid, v);
   }
   E3_HD void stage_b(int c, int tid) const {
@@ -1033,29 +1134,34 @@ struct FastZI {
   }
   // epilogue of component c alone (not HOLD): y_c = u_c - t_c or u_c - conj(chi_c) t_c
   E3_HD void epilogue_one(int c, int tid, St& s) const {
+    KAcc acc{};
+    epilogue_one(c, tid, s, acc);
+  }
+  // the epilogue of component c with fused Krylov sums: the field and r_hat
+  // values of the thread are loaded before its first store
+  E3_HD void epilogue_one_kry(int c, int tid, St& s, KAcc& acc) const {
     int j = tid % R2, pl = tid / R2;
     E3_OPAQUE(j);
     E3_OPAQUE(pl);
     const long long p = p0 + pl;
     if (p >= a.P) return;
-    cpx<T>* out = (cpx<T>*)a.out + (size_t)b * a.out_bs;
-    const cpx<T>* src = (const cpx<T>*)a.u + (size_t)b * a.out_bs;
-    const cpx<T>* chi = (const cpx<T>*)a.chi;
-    const bool herm = a.epi == EPI_SUB_CHIH && a.chi_kind != CHI_NONE;
-#pragma unroll
-    for (int bb = 0; bb < NB; ++bb) {
-#pragma unroll
-      for (int m = 0; m < MH; ++m) {
-        const int z = j + bb * R2 + m * R1;
-        cpx<T> t = s.y[0][bb][m];
-        if (herm) {
-          const long long cell = p * a.n_out + z;
-          t = cmulc(ld_nt<E3_Z_NT>(a.chi_kind == CHI_ISO ? &chi[cell] : &chi[(size_t)c * a.ncell + cell]), t);
-        }
-        const size_t o = (size_t)c * a.out_cs + (size_t)p * a.out_ps + z;
-        st_nt<E3_Z_NT_ST>(&out[o], ld_nt<E3_Z_NT>(&src[o]) - t);
-      }
-    }
+    const size_t row = (size_t)b * a.out_bs + (size_t)c * a.out_cs + (size_t)p * a.out_ps;
+    kry_epilogue_row<T, NB, MH, R1, R2>((cpx<T>*)a.out + row, (const cpx<T>*)a.u + row,
+                                        a.k.op == KRY_DOT ? (const cpx<T>*)a.k.rh + row : nullptr, nullptr, a.k.op,
+                                        j, s.y[0], acc);
+  }
+  E3_HD void epilogue_one(int c, int tid, St& s, KAcc& acc) const {
+    int j = tid % R2, pl = tid / R2;
+    E3_OPAQUE(j);
+    E3_OPAQUE(pl);
+    const long long p = p0 + pl;
+    if (p >= a.P) return;
+    const size_t row = (size_t)b * a.out_bs + (size_t)c * a.out_cs + (size_t)p * a.out_ps;
+    const cpx<T>* chi = nullptr;
+    if (a.epi == EPI_SUB_CHIH && a.chi_kind != CHI_NONE)
+      chi = (const cpx<T>*)a.chi + (a.chi_kind == CHI_ISO ? 0 : (size_t)c * a.ncell) + (size_t)p * a.n_out;
+    kry_epilogue_row<T, NB, MH, R1, R2>((cpx<T>*)a.out + row, (const cpx<T>*)a.u + row, nullptr, chi, KRY_NONE, j,
+                                        s.y[0], acc);
   }
   E3_HD void epilogue(int tid, St& s) const {
     int j = tid % R2, pl = tid / R2;

@@ -58,17 +58,49 @@ constexpr bool zf_split() {
   return E3_ZF_SPLIT || sizeof(typename K::Real) == 8;
 }
 
-template <class K>
+// Block sum of the fused Krylov partials (every thread of the block calls
+// it): thread 0 writes the block's record, block 0 the blocks per right-hand
+// side.  The grid holds gridDim.x / batch consecutive blocks per right-hand side.
+__device__ __forceinline__ void kry_block_store(const ZArgs& a, KAcc& acc) {
+  __shared__ double red[16][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double v = acc.v[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[w][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const long long per = gridDim.x / a.batch;
+    const long long b = blockIdx.x / per;
+    double* o = a.k.part + ((size_t)b * a.k.part_stride + (blockIdx.x - b * per)) * 4;
+    const int nw = (int)((blockDim.x + 63) >> 6);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < nw; ++k) s += red[k][i];
+      o[i] = s;
+    }
+    if (blockIdx.x == 0) *a.k.count = (int)per;
+  }
+}
+
+// KRY: the forward pass with a fused Krylov prologue (ZArgs::k)
+template <class K, bool KRY>
 __global__ __launch_bounds__(K::NT) void k_fast_zf(ZArgs a, long long nblk) {
   if (a.skip && *a.skip) return;
+  static_assert(!KRY || K::NT % 64 == 0, "fused reductions take whole wavefronts");
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
+  KAcc acc{};
   if constexpr (zf_split<K>()) {
     const long long bid = blockIdx.x / a.C;
     const int c = (int)(blockIdx.x - bid * a.C);
     const int b = (int)(bid / nblk);
     K k(a, bid - (long long)b * nblk, b, lds);
-    k.stage_a(c, threadIdx.x);
+    if constexpr (KRY) k.stage_a(c, threadIdx.x, acc);
+    else k.stage_a(c, threadIdx.x);
     __syncthreads();
     k.stage_b(c, threadIdx.x);
   } else {
@@ -76,32 +108,41 @@ __global__ __launch_bounds__(K::NT) void k_fast_zf(ZArgs a, long long nblk) {
     const int b = (int)(bid / nblk);
     K k(a, bid - (long long)b * nblk, b, lds);
     for (int c = 0; c < a.C; ++c) {
-      k.stage_a(c, threadIdx.x);
+      if constexpr (KRY) k.stage_a(c, threadIdx.x, acc);
+      else k.stage_a(c, threadIdx.x);
       __syncthreads();
       k.stage_b(c, threadIdx.x);
       if (c + 1 < a.C) __syncthreads();
     }
   }
+  if constexpr (KRY) {
+    if (a.k.op == KRY_S) kry_block_store(a, acc);   // KRY_P has no sums
+  }
 }
 
-template <class K, int C>
+// KRY: the inverse pass with fused Krylov sums in its epilogue (ZArgs::k)
+template <class K, int C, bool KRY>
 __global__ __launch_bounds__(K::NT) void k_fast_zi(ZArgs a, long long nblk) {
   if (a.skip && *a.skip) return;
+  static_assert(!KRY || (K::NT % 64 == 0 && !K::HOLD), "fused sums: whole wavefronts, per-component epilogue");
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
   const long long bid = blockIdx.x;
   const int b = (int)(bid / nblk);
   K k(a, bid - (long long)b * nblk, b, lds);
   typename K::St st;
+  KAcc acc{};
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     k.stage_a(c, threadIdx.x);
     __syncthreads();
     k.stage_b(c, threadIdx.x, st);
-    if constexpr (!K::HOLD) k.epilogue_one(c, threadIdx.x, st);
+    if constexpr (KRY) k.epilogue_one_kry(c, threadIdx.x, st, acc);
+    else if constexpr (!K::HOLD) k.epilogue_one(c, threadIdx.x, st);
     if (c + 1 < C) __syncthreads();
   }
   if constexpr (K::HOLD) k.epilogue(threadIdx.x, st);
+  if constexpr (KRY) kry_block_store(a, acc);
 }
 
 struct HipFastRun {
@@ -124,14 +165,25 @@ struct HipFastRun {
   void zf(const ZArgs& a) {
     const long long nblk = (a.P + K::P - 1) / K::P;
     const long long split = zf_split<K>() ? a.C : 1;
-    hipLaunchKernelGGL(k_fast_zf<K>, dim3((unsigned)(nblk * a.batch * split)), dim3(K::NT), 0, st, a, nblk);
+    const dim3 grid((unsigned)(nblk * a.batch * split));
+    if (a.k.op == KRY_P || a.k.op == KRY_S)
+      hipLaunchKernelGGL((k_fast_zf<K, true>), grid, dim3(K::NT), 0, st, a, nblk);
+    else
+      hipLaunchKernelGGL((k_fast_zf<K, false>), grid, dim3(K::NT), 0, st, a, nblk);
     err = hipGetLastError();
   }
   template <class K>
   void zi(const ZArgs& a) {
     const long long nblk = (a.P + K::P - 1) / K::P;
-    hipLaunchKernelGGL((k_fast_zi<K, K::NCOMP>), dim3((unsigned)(nblk * a.batch)), dim3(K::NT), 0, st,
-                       a, nblk);
+    const dim3 grid((unsigned)(nblk * a.batch));
+    if constexpr (!K::HOLD) {
+      if (a.k.op == KRY_DOT || a.k.op == KRY_DOT2) {
+        hipLaunchKernelGGL((k_fast_zi<K, K::NCOMP, true>), grid, dim3(K::NT), 0, st, a, nblk);
+        err = hipGetLastError();
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_fast_zi<K, K::NCOMP, false>), grid, dim3(K::NT), 0, st, a, nblk);
     err = hipGetLastError();
   }
 };

@@ -103,10 +103,18 @@ void s_pass(Launcher& go, int prec, const void* tw, const AxisPlan& ax, SArgs s)
   go.s(s, ncb, s.Q, prec, lds_bytes(s.T, s.ps, prec));
 }
 
+// Passes of run_apply (bit i: pass P(i+1)); a solver runs P1 and P2..P5 of
+// one matvec as two calls when a decision falls between them.
+enum : unsigned { PASSES_ALL = 0x1Fu, PASSES_FIRST = 0x01u, PASSES_REST = 0x1Eu };
+
 // y = A u (adjoint = false) or y = A^H u (adjoint = true) for all batch RHS.
+// k1 / k5: fused Krylov steps of the first / last z pass (device BiCGStab);
+// skip: device flag of every pass run here (skip1, when given, of P1).
 template <class Launcher>
 void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, int chi_kind,
-               const void* u, void* y, void* work, bool adjoint, const int* skip = nullptr) {
+               const void* u, void* y, void* work, bool adjoint, const int* skip = nullptr,
+               const ZKry* k1 = nullptr, const ZKry* k5 = nullptr, const int* skip1 = nullptr,
+               unsigned passes = PASSES_ALL) {
   const int prec = p.prec;
   const size_t es = elem_bytes(prec);
   void* W1 = work;
@@ -128,9 +136,11 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.out_ps = Lz; z.out_cs = P * Lz; z.out_bs = C * P * Lz;
   z.n_in = Nz; z.n_out = Lz; z.inv = 0;
   z.pro = adjoint ? PRO_COPY : PRO_CHI; z.epi = EPI_STORE;
-  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.perm = 1; z.skip = skip; z.variant = p.variant[PASS_Z];
+  z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.perm = 1; z.skip = skip1 ? skip1 : skip;
+  z.variant = p.variant[PASS_Z];
   z.nt = nt;
-  z_pass(go, p, prec, twz, p.ax[2], z);
+  if (k1) z.k = *k1;
+  if (passes & 0x01u) z_pass(go, p, prec, twz, p.ax[2], z);
 
   // P2: y forward (Ny live rows -> Ly)
   SArgs s;
@@ -138,7 +148,7 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   s.in = W1; s.out = W2; s.Q = (long long)p.batch * C * Nx; s.W = Lz;
   s.in_qs = (long long)Ny * Lz; s.out_qs = (long long)Ly * Lz;
   s.n_in = Ny; s.n_out = Ly; s.inv = 0; s.skip = skip; s.variant = p.variant[PASS_S]; s.nt = nt;
-  s_pass(go, prec, twy, p.ax[1], s);
+  if (passes & 0x02u) s_pass(go, prec, twy, p.ax[1], s);
 
   // P3: x forward, spectral multiply, x inverse (Nx rows kept), in place
   XArgs x;
@@ -151,12 +161,12 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   x.ps = Lx + 1;
   x.T = tile_count(lds_bytes(p.C, x.ps, prec), 16);
   if (x.T > x.W) x.T = (int)x.W;
-  go.x(x, (x.W + x.T - 1) / x.T, p.batch, prec, lds_bytes(p.C * x.T, x.ps, prec));
+  if (passes & 0x04u) go.x(x, (x.W + x.T - 1) / x.T, p.batch, prec, lds_bytes(p.C * x.T, x.ps, prec));
 
   // P4: y inverse (Ny rows kept)
   s.in = W2; s.out = W1; s.in_qs = (long long)Ly * Lz; s.out_qs = (long long)Ny * Lz;
   s.n_in = Ly; s.n_out = Ny; s.inv = 1;
-  s_pass(go, prec, twy, p.ax[1], s);
+  if (passes & 0x08u) s_pass(go, prec, twy, p.ax[1], s);
 
   // P5: z inverse, Nz kept, epilogue y = u - t or y = v - chi^H t
   std::memset(&z, 0, sizeof(z));
@@ -166,7 +176,8 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.n_in = Lz; z.n_out = Nz; z.inv = 1;
   z.pro = PRO_COPY; z.epi = adjoint ? EPI_SUB_CHIH : EPI_SUB;
   z.chi = chi; z.chi_kind = chi_kind; z.ncell = P * Nz; z.u = u; z.perm = 1; z.skip = skip; z.variant = p.variant[PASS_Z];
-  z_pass(go, p, prec, twz, p.ax[2], z);
+  if (k5) z.k = *k5;
+  if (passes & 0x10u) z_pass(go, p, prec, twz, p.ax[2], z);
 }
 
 // Octant spectrum of the operator kernel, built in FP64 then cast.

@@ -216,6 +216,41 @@ __global__ __launch_bounds__(kRT) void k_ur(long long n, cpx<T>* u, const cpx<T>
   if (threadIdx.x == 0) part[((size_t)b * kRB + blockIdx.x) * 4] = acc;
 }
 
+// Fused form of k_ur followed by the next iteration's (r_hat, r): u_alpha =
+// u + alpha p; then (no halt) u = u_alpha + omega s, r = s - omega t, and the
+// partials (r_hat, r) in entries 0-1 and |r|^2 in entry 2.
+template <typename T>
+__global__ __launch_bounds__(kRT) void k_ur_dot(long long n, cpx<T>* u, const cpx<T>* p, const cpx<T>* s, cpx<T>* r,
+                                                const cpx<T>* t, const cpx<T>* rh, const BiState* st, double* part) {
+  const int b = blockIdx.y;
+  if (st[b].done) return;
+  const long long off = (long long)b * n;
+  u += off; p += off; s += off; r += off; t += off; rh += off;
+  const bool h = st[b].halt != 0;
+  const cpx<T> al = cast<T>(st[b].alpha), om = cast<T>(st[b].omega);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const cpx<T> ua = u[i] + al * p[i];
+    if (h) {
+      u[i] = ua;
+    } else {
+      const cpx<T> si = s[i];
+      u[i] = ua + om * si;
+      const cpx<T> x = si - om * t[i];
+      r[i] = x;
+      const cpx<T> q = rh[i];
+      acc[0] += (double)q.re * x.re + (double)q.im * x.im;
+      acc[1] += (double)q.re * x.im - (double)q.im * x.re;
+      acc[2] += (double)x.re * x.re + (double)x.im * x.im;
+    }
+  }
+  bsum<3>(acc);
+  if (threadIdx.x == 0) {
+    double* o = part + ((size_t)b * kRB + blockIdx.x) * 4;
+    o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2];
+  }
+}
+
 // ---- decide kernels: one block per RHS ----------------------------------------
 struct Hist {
   double* h;
@@ -223,10 +258,24 @@ struct Hist {
   int cap;
 };
 
-__device__ __forceinline__ const double* part_of(const double* part) { return part + (size_t)blockIdx.x * kRB * 4; }
+// Where a decide kernel finds its partial sums: records of 4 doubles, `stride`
+// records per RHS, the first np (or *count, when given) used, from entry off.
+struct PartSrc {
+  const double* part;
+  long long stride;
+  int np;
+  const int* count;
+  int off;
+};
+
+__host__ __device__ inline PartSrc vec_parts(const double* part, int np, int off = 0) {
+  return PartSrc{part, kRB, np, nullptr, off};
+}
 
 template <int NV>
-__device__ __forceinline__ void final_sum4(const double* part, int np, double* out) {
+__device__ __forceinline__ void final_sum4(PartSrc ps, double* out) {
+  const double* part = ps.part + (size_t)blockIdx.x * ps.stride * 4 + ps.off;
+  const int np = ps.count ? *ps.count : ps.np;
   double v[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) v[i] = 0.0;
@@ -238,6 +287,23 @@ __device__ __forceinline__ void final_sum4(const double* part, int np, double* o
   for (int i = 0; i < NV; ++i) out[i] = v[i];
 }
 
+// The coefficients the fused z passes read (KryCoef), written by thread 0 of
+// a decide kernel when it leaves (kc null: the stand-alone vector kernels run).
+struct Publish {
+  const BiState* st;
+  KryCoef* kc;
+  __device__ ~Publish() {
+    if (!kc) return;
+    KryCoef c;
+    c.beta[0] = st->beta.re; c.beta[1] = st->beta.im;
+    c.omega[0] = st->omega.re; c.omega[1] = st->omega.im;
+    c.alpha[0] = st->alpha.re; c.alpha[1] = st->alpha.im;
+    c.first = st->it == 0 ? 1 : 0;
+    c.done = st->done;
+    *kc = c;
+  }
+};
+
 __device__ __forceinline__ void record_b(BiState* st, Hist H, double v) {
   st->last_rel = v;
   if (st->nhist < st->hist_cap) {
@@ -247,12 +313,13 @@ __device__ __forceinline__ void record_b(BiState* st, Hist H, double v) {
   st->nhist += 1;
 }
 
-__global__ __launch_bounds__(kRT) void k_rho(const double* part, int np, BiState* sts) {
+__global__ __launch_bounds__(kRT) void k_rho(PartSrc ps, BiState* sts, KryCoef* kcs) {
   BiState* st = sts + blockIdx.x;
   if (st->done) return;
   double v[2];
-  final_sum4<2>(part_of(part), np, v);
+  final_sum4<2>(ps, v);
   if (threadIdx.x) return;
+  Publish pub{st, kcs ? kcs + blockIdx.x : nullptr};
   const cpx<double> rho = mk(v[0], v[1]);
   st->rho = rho;
   if (cabs_d(rho) == 0.0) return finish(st, ST_BREAKDOWN_RHO);
@@ -263,23 +330,24 @@ __global__ __launch_bounds__(kRT) void k_rho(const double* part, int np, BiState
   if (st->use_max_act && st->mv + 1 > st->max_act) return finish(st, ST_MAX_ACTIONS);
 }
 
-__global__ __launch_bounds__(kRT) void k_alpha(const double* part, int np, BiState* sts) {
+__global__ __launch_bounds__(kRT) void k_alpha(PartSrc ps, BiState* sts, KryCoef* kcs) {
   BiState* st = sts + blockIdx.x;
   if (st->done) return;
   double v[2];
-  final_sum4<2>(part_of(part), np, v);
+  final_sum4<2>(ps, v);
   if (threadIdx.x) return;
+  Publish pub{st, kcs ? kcs + blockIdx.x : nullptr};
   st->mv += 1;
   const cpx<double> den = mk(v[0], v[1]);
   if (cabs_d(den) == 0.0) return finish(st, ST_BREAKDOWN_ALPHA);
   st->alpha = cdiv(st->rho, den);
 }
 
-__global__ __launch_bounds__(kRT) void k_sfin(const double* part, int np, BiState* sts, Hist H) {
+__global__ __launch_bounds__(kRT) void k_sfin(PartSrc ps, BiState* sts, Hist H) {
   BiState* st = sts + blockIdx.x;
   if (st->done) return;
   double v;
-  final_sum4<1>(part_of(part), np, &v);
+  final_sum4<1>(ps, &v);
   if (threadIdx.x) return;
   const double sr = sqrt(v) / st->rhs_norm;
   st->s_rel = sr;
@@ -294,11 +362,11 @@ __global__ __launch_bounds__(kRT) void k_sfin(const double* part, int np, BiStat
   }
 }
 
-__global__ __launch_bounds__(kRT) void k_omega(const double* part, int np, BiState* sts, Hist H) {
+__global__ __launch_bounds__(kRT) void k_omega(PartSrc ps, BiState* sts, Hist H) {
   BiState* st = sts + blockIdx.x;
   if (st->skip_t) return;
   double v[4];
-  final_sum4<4>(part_of(part), np, v);
+  final_sum4<4>(ps, v);
   if (threadIdx.x) return;
   st->mv += 1;
   const cpx<double> tt = mk(v[0], v[1]);
@@ -313,12 +381,13 @@ __global__ __launch_bounds__(kRT) void k_omega(const double* part, int np, BiSta
   }
 }
 
-__global__ __launch_bounds__(kRT) void k_rfin(const double* part, int np, BiState* sts, Hist H) {
+__global__ __launch_bounds__(kRT) void k_rfin(PartSrc ps, BiState* sts, Hist H, KryCoef* kcs) {
   BiState* st = sts + blockIdx.x;
   if (st->done) return;
   double v;
-  final_sum4<1>(part_of(part), np, &v);
+  final_sum4<1>(ps, &v);
   if (threadIdx.x) return;
+  Publish pub{st, kcs ? kcs + blockIdx.x : nullptr};
   st->it += 1;
   if (st->halt) {
     st->done = 1;
@@ -360,7 +429,8 @@ __global__ __launch_bounds__(kRT) void k_group(const BiState* sts, int nb, Group
 // Start state of every RHS from the norm partials of f (r = f - A 0 = f, A 0
 // counted as one action) and its first history entry.  One thread per RHS
 // sums the nb partials in order, as a host loop would.
-__global__ void k_init(const double* part, int nb, int B, em3d_bicgstab_params prm, BiState* sts, Hist H) {
+__global__ void k_init(const double* part, int nb, int B, em3d_bicgstab_params prm, BiState* sts, Hist H,
+                       KryCoef* kcs) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   double nn = 0.0;
@@ -401,6 +471,7 @@ __global__ void k_init(const double* part, int nb, int B, em3d_bicgstab_params p
   sts[b] = s;
   H.h[(size_t)b * H.cap] = rel0;
   H.a[(size_t)b * H.cap] = s.mv;
+  if (kcs) Publish{&sts[b], kcs + b};
 }
 
 int sfail(const std::string& m) { return set_error(m); }
@@ -418,6 +489,7 @@ long long field_elems(const em3d_plan_s* plan) {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // solver scratch: r, r_hat, p, v, s, t (batch fields each) | partials | states | group | rel0 (unused)
+// | fused-step coefficients | z-pass partials of P1 and P5 | their block counts
 // | preconditioned p and s (preconditioned solves only)
 struct Work {
   void* r;
@@ -430,20 +502,30 @@ struct Work {
   BiState* st;
   Group* g;
   double* rel0;
+  KryCoef* kc;
+  double* zpart1;
+  double* zpart5;
+  int* zcount;
+  long long zstride;
   void* ph;
   void* sh;
 };
 
 size_t part_bytes(int B) { return align256((size_t)B * kRB * 4 * sizeof(double)); }
 
-size_t work_total(size_t vb, int B);
+// records per RHS of the z-pass partials: at most one block per pencil and component
+long long zpart_stride(const Plan& p) { return (long long)p.C * p.N[0] * p.N[1]; }
+size_t zpart_bytes(const Plan& p) { return align256((size_t)p.batch * zpart_stride(p) * 4 * sizeof(double)); }
 
-Work carve(void* base, size_t vb, int B) {
+size_t work_total(const Plan& p, size_t vb);
+
+Work carve(const Plan& p, void* base, size_t vb) {
+  const int B = p.batch;
   Work w;
   char* c = (char*)base;
   const size_t a = align256(vb);
   w.r = c; w.rh = c + a; w.p = c + 2 * a; w.v = c + 3 * a; w.s = c + 4 * a; w.t = c + 5 * a;
-  w.ph = (char*)base + work_total(vb, B);
+  w.ph = (char*)base + work_total(p, vb);
   w.sh = (char*)w.ph + a;
   c += 6 * a;
   w.part = (double*)c;
@@ -453,12 +535,66 @@ Work carve(void* base, size_t vb, int B) {
   w.g = (Group*)c;
   c += align256(sizeof(Group));
   w.rel0 = (double*)c;
+  c += align256((size_t)B * sizeof(double));
+  w.kc = (KryCoef*)c;
+  c += align256((size_t)B * sizeof(KryCoef));
+  w.zpart1 = (double*)c;
+  c += zpart_bytes(p);
+  w.zpart5 = (double*)c;
+  c += zpart_bytes(p);
+  w.zcount = (int*)c;
+  w.zstride = zpart_stride(p);
   return w;
 }
 
-size_t work_total(size_t vb, int B) {
+size_t work_total(const Plan& p, size_t vb) {
+  const int B = p.batch;
   return 6 * align256(vb) + part_bytes(B) + align256((size_t)B * sizeof(BiState)) + align256(sizeof(Group)) +
-         align256((size_t)B * sizeof(double));
+         align256((size_t)B * sizeof(double)) + align256((size_t)B * sizeof(KryCoef)) + 2 * zpart_bytes(p) +
+         align256(2 * sizeof(int));
+}
+
+// One BiCGStab iteration with the vector updates fused into the matvecs'
+// z passes (ZKry): p = r + beta (p - omega v) in the first pass of v = A p and
+// (r_hat, v) in its last; s = r - alpha v and |s|^2 in the first pass of
+// t = A s, (t, t) and (t, s) in its last; then one field kernel for u and r
+// with the next (r_hat, r) and |r|^2.  Same decisions as enqueue_iteration.
+template <typename T>
+void enqueue_iteration_fused(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, void* u, void* work,
+                             const Work& w, long long n, int nb, int B, Hist H, hipStream_t stream, int* fail) {
+  const dim3 gv(nb, B), gd(B);
+  const int* all_done = &w.g->all_done;
+  const int* all_skip_t = &w.g->all_skip_t;
+  const PartSrc z1{w.zpart1, w.zstride, 0, &w.zcount[0], 0};
+  const PartSrc z5{w.zpart5, w.zstride, 0, &w.zcount[1], 0};
+  hipLaunchKernelGGL(k_rho, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, w.kc);
+  hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
+  ZKry kp{KRY_P, w.r, w.v, w.p, nullptr, w.kc, w.zpart1, w.zstride, &w.zcount[0]};
+  ZKry kd{KRY_DOT, nullptr, nullptr, nullptr, w.rh, w.kc, w.zpart5, w.zstride, &w.zcount[1]};
+  if (apply_plan(plan, khat, chi, chi_kind, w.p, w.v, work, stream, false, "em3d_bicgstab matvec", all_done, &kp,
+                 &kd)) {
+    *fail = 1;
+    return;
+  }
+  hipLaunchKernelGGL(k_alpha, gd, dim3(kRT), 0, stream, z5, w.st, w.kc);
+  ZKry ks{KRY_S, w.r, w.v, w.s, nullptr, w.kc, w.zpart1, w.zstride, &w.zcount[0]};
+  ZKry kt{KRY_DOT2, nullptr, nullptr, nullptr, nullptr, w.kc, w.zpart5, w.zstride, &w.zcount[1]};
+  if (apply_plan(plan, khat, chi, chi_kind, w.s, w.t, work, stream, false, "em3d_bicgstab matvec", all_skip_t, &ks,
+                 &kt, all_done, PASSES_FIRST)) {
+    *fail = 1;
+    return;
+  }
+  hipLaunchKernelGGL(k_sfin, gd, dim3(kRT), 0, stream, z1, w.st, H);
+  hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
+  if (apply_plan(plan, khat, chi, chi_kind, w.s, w.t, work, stream, false, "em3d_bicgstab matvec", all_skip_t, &ks,
+                 &kt, all_done, PASSES_REST)) {
+    *fail = 1;
+    return;
+  }
+  hipLaunchKernelGGL(k_omega, gd, dim3(kRT), 0, stream, z5, w.st, H);
+  hipLaunchKernelGGL(k_ur_dot<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)u, (const cpx<T>*)w.p, (const cpx<T>*)w.s,
+                     (cpx<T>*)w.r, (const cpx<T>*)w.t, (const cpx<T>*)w.rh, w.st, w.part);
+  hipLaunchKernelGGL(k_rfin, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb, 2), w.st, H, w.kc);
 }
 
 template <typename T>
@@ -472,7 +608,7 @@ void enqueue_iteration(em3d_plan_t plan, const void* khat, const void* chi, int 
   void* pu = pc ? w.ph : w.p;   // search directions the matvecs and the update use
   void* su = pc ? w.sh : w.s;
   hipLaunchKernelGGL(k_dot<T>, gv, dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.r, w.part, w.st, 0);
-  hipLaunchKernelGGL(k_rho, gd, dim3(kRT), 0, stream, w.part, nb, w.st);
+  hipLaunchKernelGGL(k_rho, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, (KryCoef*)nullptr);
   hipLaunchKernelGGL(k_p<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)w.p, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st);
   hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
   if (pc && fno_precond_apply(pc, B, P.C, P.N[0], P.N[1], P.N[2], P.prec, w.p, w.ph, all_done, stream)) {
@@ -484,10 +620,10 @@ void enqueue_iteration(em3d_plan_t plan, const void* khat, const void* chi, int 
     return;
   }
   hipLaunchKernelGGL(k_dot<T>, gv, dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.v, w.part, w.st, 0);
-  hipLaunchKernelGGL(k_alpha, gd, dim3(kRT), 0, stream, w.part, nb, w.st);
+  hipLaunchKernelGGL(k_alpha, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, (KryCoef*)nullptr);
   hipLaunchKernelGGL(k_s<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)w.s, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st,
                      w.part);
-  hipLaunchKernelGGL(k_sfin, gd, dim3(kRT), 0, stream, w.part, nb, w.st, H);
+  hipLaunchKernelGGL(k_sfin, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, H);
   hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
   if (pc && fno_precond_apply(pc, B, P.C, P.N[0], P.N[1], P.N[2], P.prec, w.s, w.sh, all_skip_t, stream)) {
     *fail = 1;
@@ -498,10 +634,10 @@ void enqueue_iteration(em3d_plan_t plan, const void* khat, const void* chi, int 
     return;
   }
   hipLaunchKernelGGL(k_dot2<T>, gv, dim3(kRT), 0, stream, n, (const cpx<T>*)w.t, (const cpx<T>*)w.s, w.part, w.st);
-  hipLaunchKernelGGL(k_omega, gd, dim3(kRT), 0, stream, w.part, nb, w.st, H);
+  hipLaunchKernelGGL(k_omega, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, H);
   hipLaunchKernelGGL(k_ur<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)u, (const cpx<T>*)pu, (const cpx<T>*)su,
                      (const cpx<T>*)w.s, (cpx<T>*)w.r, (const cpx<T>*)w.t, w.st, w.part);
-  hipLaunchKernelGGL(k_rfin, gd, dim3(kRT), 0, stream, w.part, nb, w.st, H);
+  hipLaunchKernelGGL(k_rfin, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, H, (KryCoef*)nullptr);
 }
 
 }  // namespace
@@ -511,7 +647,7 @@ extern "C" {
 size_t em3d_bicgstab_work_bytes(em3d_plan_t plan) {
   if (!plan) return 0;
   const int B = plan->p.batch;
-  return work_total((size_t)B * field_elems(plan) * elem_bytes(plan->p.prec), B);
+  return work_total(plan->p, (size_t)B * field_elems(plan) * elem_bytes(plan->p.prec));
 }
 
 size_t em3d_bicgstab_pc_work_bytes(em3d_plan_t plan) {
@@ -537,7 +673,10 @@ int bicgstab_impl(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
   const long long n = field_elems(plan);
   const size_t es = elem_bytes(prec);
   const size_t vb = (size_t)B * n * es;
-  Work w = carve(solver_work, vb, B);
+  Work w = carve(P, solver_work, vb);
+  // fused vector steps: the matvec contrast must not mix components, and the
+  // preconditioned form keeps its own directions
+  const bool fused = prm->fused != 0 && !pc && chi_kind != CHI_FULL;
   const int nb = (int)std::min<long long>(kRB, (n + kRT - 1) / kRT);
   const Hist H{history, action_counts, history_cap};
   hipError_t e;
@@ -557,7 +696,8 @@ int bicgstab_impl(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
       (e = hipMemcpyAsync(w.rh, rhs, vb, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
       (e = hipMemsetAsync(w.p, 0, vb, stream)) != hipSuccess || (e = hipMemsetAsync(w.v, 0, vb, stream)) != hipSuccess)
     return chk(e, "init");
-  hipLaunchKernelGGL(k_init, dim3((B + 63) / 64), dim3(64), 0, stream, (const double*)w.part, nb, B, *prm, w.st, H);
+  hipLaunchKernelGGL(k_init, dim3((B + 63) / 64), dim3(64), 0, stream, (const double*)w.part, nb, B, *prm, w.st, H,
+                     fused ? w.kc : (KryCoef*)nullptr);
   if ((e = hipGetLastError()) != hipSuccess) return chk(e, "init launch");
 
   // chunks of iterations, the states read back after each (a solve that is
@@ -568,7 +708,11 @@ int bicgstab_impl(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
   for (int it0 = 0; it0 < prm->max_iter; it0 += chunk) {
     for (int k = 0; k < chunk && it0 + k < prm->max_iter; ++k) {
       int fail = 0;
-      if (prec == PREC_DOUBLE)
+      if (fused && prec == PREC_DOUBLE)
+        enqueue_iteration_fused<double>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, stream, &fail);
+      else if (fused)
+        enqueue_iteration_fused<float>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, stream, &fail);
+      else if (prec == PREC_DOUBLE)
         enqueue_iteration<double>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, pc, stream, &fail);
       else
         enqueue_iteration<float>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, pc, stream, &fail);

@@ -391,7 +391,7 @@ class _FieldApplier:
         prm = _lib.BiCGStabParams(int(cfg.max_iter), float(cfg.rtol), float(cfg.atol),
                                   -1.0 if cfg.divergence_guard is None else float(cfg.divergence_guard),
                                   -1 if cfg.max_operator_actions is None else int(cfg.max_operator_actions),
-                                  int(getattr(cfg, "chunk", 4) or 4))
+                                  int(getattr(cfg, "chunk", 4) or 4), 1 if getattr(cfg, "fused", True) else 0)
         res = (_lib.SolverResultC * batch)()
         args = (plan.handle, khat.data_ptr(), chi.data_ptr() if chi is not None else None, chi_kind, t.data_ptr(),
                 u.data_ptr(), self._workspace(plan).data_ptr(), sw.data_ptr(), C.byref(prm), hist.data_ptr(),

@@ -16,6 +16,7 @@ class SolverConfig:
     mu: Optional[complex] = None
     radius: Optional[float] = None
     chunk: int = 4          # device solvers: iterations enqueued between host checks
+    fused: bool = True      # device BiCGStab: vector updates fused into the matvec passes where the contrast allows
 
     def require_gamma(self) -> None:
         if self.mu is None or self.radius is None:

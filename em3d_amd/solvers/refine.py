@@ -104,7 +104,7 @@ class MixedBiCGStab:
             # solve no further than the remaining reduction asks (with a margin)
             want = max(self.inner_rtol, 0.5 * tol / rel)
             icfg = SolverConfig(max_iter=budget if pc is None else min(budget, self.preconditioned_max_iter),
-                                rtol=min(want, 0.5), chunk=cfg.chunk)
+                                rtol=min(want, 0.5), chunk=cfg.chunk, fused=getattr(cfg, "fused", True))
             saved = (x.clone(), r.clone()) if pc is not None else None   # to undo an unhelpful partial step
             ires = comp.device_bicgstab(r32, icfg, precond=pc)
             inner_it += ires.iterations
@@ -140,7 +140,8 @@ class MixedBiCGStab:
         fallback = None
         if status in ("stagnated", "max_iter") and math.isfinite(rel):
             # plain FP64 BiCGStab on the remaining correction A d = r
-            fcfg = SolverConfig(max_iter=cfg.max_iter, rtol=0.5 * tol / rel, chunk=cfg.chunk)
+            fcfg = SolverConfig(max_iter=cfg.max_iter, rtol=0.5 * tol / rel, chunk=cfg.chunk,
+                                fused=getattr(cfg, "fused", True))
             fres = operator.device_bicgstab(r, fcfg)
             fallback = fres.status
             inner_it += fres.iterations

@@ -149,6 +149,10 @@ typedef struct {
   double divergence_guard;
   long long max_operator_actions;
   int chunk;
+  /* 1: the p and s updates and the dot products run inside the matvecs' z
+   * passes and the u/r update carries the next (r_hat, r) (isotropic or
+   * diagonal contrast, no preconditioner); 0: stand-alone vector kernels */
+  int fused;
 } em3d_bicgstab_params;
 
 /* status: 0 converged, 1 nonfinite, 2 max_operator_actions, 3 breakdown_rho,

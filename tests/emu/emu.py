@@ -38,6 +38,9 @@ def load(path: str | None = None):
     lib.emu_kernel_build.argtypes = [C.c_void_p] + [C.c_double] * 4 + [C.c_void_p] * 2
     lib.emu_apply.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                               C.c_void_p, C.c_int]
+    lib.emu_apply_kry.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p]
     lib.emu_last_error.restype = C.c_char_p
     lib.emu_plan_variant_count.argtypes = [C.c_void_p, C.c_int]
     lib.emu_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -109,3 +112,29 @@ def pack_chi(chi, ncomp, cdt, kind=None):
     if kind == 2:
         return 2, np.ascontiguousarray(np.stack([chi[a, a] for a in range(3)]), dtype=cdt)
     return 3, np.ascontiguousarray(chi, dtype=cdt)
+
+
+KRY_NONE, KRY_P, KRY_S, KRY_DOT, KRY_DOT2 = 0, 1, 2, 3, 4
+KRY_COEF = np.dtype([("beta", np.float64, 2), ("omega", np.float64, 2), ("alpha", np.float64, 2),
+                     ("first", np.int32), ("done", np.int32)])
+
+
+def apply_kry(op, u, op1, r, v, op5, rh, coef, stride=4096):
+    """y = A u with the fused Krylov steps (u updated in place by op1);
+    returns (y, per-RHS sums of the P1 partials, of the P5 partials)."""
+    u = np.array(u, dtype=op.cdt, order="C", copy=True)     # updated in place by op1
+    r = np.ascontiguousarray(r, dtype=op.cdt)
+    v = np.ascontiguousarray(v, dtype=op.cdt)
+    rh = np.ascontiguousarray(rh, dtype=op.cdt)
+    coef = np.ascontiguousarray(coef, dtype=KRY_COEF)
+    y = np.empty_like(u)
+    B = op.batch
+    p1 = np.full((B, stride, 4), np.nan)
+    p5 = np.full((B, stride, 4), np.nan)
+    counts = np.zeros(2, np.int32)
+    op.lib.emu_apply_kry(op.h, op.khat.ctypes.data, op.chi.ctypes.data, op.chi_kind, u.ctypes.data, y.ctypes.data,
+                         op.work.ctypes.data, op1, r.ctypes.data, v.ctypes.data, op5, rh.ctypes.data,
+                         coef.ctypes.data, p1.ctypes.data, p5.ctypes.data, stride, counts.ctypes.data)
+    s1 = p1[:, :counts[0]].sum(axis=1) if op1 == KRY_S else None
+    s5 = p5[:, :counts[1]].sum(axis=1) if op5 in (KRY_DOT, KRY_DOT2) else None
+    return u, y, s1, s5

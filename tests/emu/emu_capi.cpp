@@ -29,6 +29,18 @@ void poison(std::vector<unsigned char>& mem) {
   for (size_t i = 0; i < n; ++i) p[i] = mk(nan, nan);
 }
 
+// Fused Krylov partials of one block: the threads' sums in order, as the
+// kernels' block reduction writes them (record `blk` of right-hand side b).
+void kry_store(const ZArgs& a, const std::vector<KAcc>& acc, int b, long long blk, long long per) {
+  double* o = a.k.part + ((size_t)b * a.k.part_stride + blk) * 4;
+  for (int i = 0; i < 4; ++i) {
+    double s = 0.0;
+    for (const KAcc& x : acc) s += x.v[i];
+    o[i] = s;
+  }
+  if (b == 0 && blk == 0) *a.k.count = (int)per;
+}
+
 // Replays the fast (power-of-two) kernels: per-thread register state lives
 // in an array indexed by thread id; barriers separate the phase loops.
 long long g_fast_calls[4] = {0, 0, 0, 0};   // s, x, zf, zi
@@ -82,14 +94,20 @@ struct EmuFastRun {
     using T = typename K::Real;
     ++g_fast_calls[2];
     const long long nblk = (a.P + K::P - 1) / K::P;
+    const bool kry = a.k.op == KRY_P || a.k.op == KRY_S;
     for (long long bid = 0; bid < nblk * a.batch; ++bid) {
       auto mem = tile<K>();
       const int b = (int)(bid / nblk);
       K k(a, bid - (long long)b * nblk, b, (cpx<T>*)mem.data());
+      std::vector<KAcc> acc(K::NT, KAcc{});
       for (int c = 0; c < a.C; ++c) {
-        for (int t = 0; t < K::NT; ++t) k.stage_a(c, t);
+        for (int t = 0; t < K::NT; ++t) {
+          if (kry) k.stage_a(c, t, acc[t]);
+          else k.stage_a(c, t);
+        }
         for (int t = 0; t < K::NT; ++t) k.stage_b(c, t);
       }
+      if (a.k.op == KRY_S) kry_store(a, acc, b, bid - (long long)b * nblk, nblk);
     }
   }
   template <class K>
@@ -102,14 +120,20 @@ struct EmuFastRun {
       const int b = (int)(bid / nblk);
       K k(a, bid - (long long)b * nblk, b, (cpx<T>*)mem.data());
       std::vector<typename K::St> st(K::NT);
+      std::vector<KAcc> acc(K::NT, KAcc{});
       for (int c = 0; c < a.C; ++c) {
         for (int t = 0; t < K::NT; ++t) k.stage_a(c, t);
         for (int t = 0; t < K::NT; ++t) k.stage_b(c, t, st[t]);
-        if constexpr (!K::HOLD)
-          for (int t = 0; t < K::NT; ++t) k.epilogue_one(c, t, st[t]);
+        if constexpr (!K::HOLD) {
+          for (int t = 0; t < K::NT; ++t) {
+            if (a.k.op == KRY_DOT || a.k.op == KRY_DOT2) k.epilogue_one_kry(c, t, st[t], acc[t]);
+            else k.epilogue_one(c, t, st[t]);
+          }
+        }
       }
       if constexpr (K::HOLD)
         for (int t = 0; t < K::NT; ++t) k.epilogue(t, st[t]);
+      if (a.k.op == KRY_DOT || a.k.op == KRY_DOT2) kry_store(a, acc, b, bid - (long long)b * nblk, nblk);
     }
   }
 };
@@ -126,10 +150,13 @@ struct EmuLauncher {
       poison<T>(mem);
       const int b = (int)(bid / nblk);
       ZPass<T> z(a, bid - (long long)b * nblk, b, (cpx<T>*)mem.data());
-      for (int t = 0; t < nt; ++t) z.load(t, nt);
+      std::vector<KAcc> acc(nt, KAcc{});
+      for (int t = 0; t < nt; ++t) z.load(t, nt, acc[t]);
       for (int s = 0; s < a.plan.nst; ++s)
         for (int t = 0; t < nt; ++t) z.stage(s, t, nt);
-      for (int t = 0; t < nt; ++t) z.store(t, nt);
+      for (int t = 0; t < nt; ++t) z.store(t, nt, acc[t]);
+      if (a.k.op == KRY_S || a.k.op == KRY_DOT || a.k.op == KRY_DOT2)
+        kry_store(a, acc, b, bid - (long long)b * nblk, nblk);
     }
   }
   void z(const ZArgs& a, long long nblk, int batch, int prec, size_t lds) {
@@ -266,6 +293,21 @@ void emu_apply(void* h, const void* khat, const void* chi, int chi_kind, const v
                int adjoint) {
   EmuLauncher go;
   run_apply(go, ((EmuPlan*)h)->p, khat, chi, chi_kind, u, y, work, adjoint != 0);
+}
+
+// y = A u with the fused Krylov steps of the device BiCGStab: op1 (KRY_P /
+// KRY_S, updating u in place from r and v) in the first z pass, op5 (KRY_DOT
+// / KRY_DOT2) in the last; partial sums of P1 / P5 to part1 / part5.
+void emu_apply_kry(void* h, const void* khat, const void* chi, int chi_kind, void* u, void* y, void* work,
+                   int op1, const void* r, const void* v, int op5, const void* rh, const void* coef, double* part1,
+                   double* part5, long long stride, int* counts) {
+  ZKry k1{}, k5{};
+  k1.op = op1; k1.r = r; k1.v = v; k1.w = u; k1.coef = (const KryCoef*)coef;
+  k1.part = part1; k1.part_stride = stride; k1.count = &counts[0];
+  k5.op = op5; k5.rh = rh; k5.coef = (const KryCoef*)coef;
+  k5.part = part5; k5.part_stride = stride; k5.count = &counts[1];
+  EmuLauncher go;
+  run_apply(go, ((EmuPlan*)h)->p, khat, chi, chi_kind, u, y, work, false, nullptr, &k1, &k5);
 }
 
 }  // extern "C"
