@@ -399,6 +399,35 @@ def profile_case(args):
     print(json.dumps({"case": args.case, "ms_per_matvec": round(t / args.steps * 1e3, 4)}), flush=True)
 
 
+def solve_case(args):
+    """Time-to-rtol 1e-8 at N^3 (config-2 ellipsoid, k0 = --k0): plain
+    complex128 BiCGStab and the complex64/FP64 mixed solver, each with the
+    vector updates fused into the matvec passes and with stand-alone vector
+    kernels, best of --steps runs after one warm-up; one JSON line."""
+    import torch
+
+    import em3d_amd as em
+    be64 = em.Backend.hip(em.Precision.DOUBLE)
+    prob, _ = make_problem(em, be64, args.n, args.k0)
+    op = em.Operator(prob)
+    f = prob.wave
+    op.matvec(f)
+    op.single_precision_companion().matvec(f.to(torch.complex64))
+    out = {"case": "solve", "grid": args.n, "k0": args.k0}
+    for fused in (True, False):
+        cfg = em.SolverConfig(max_iter=1000, rtol=1e-8, fused=fused)
+        for name, mk in (("fp64_bicgstab", lambda: em.BiCGStab(cfg)), ("mixed_bicgstab", lambda: em.MixedBiCGStab(cfg))):
+            best = None
+            for _ in range(max(1, args.steps) + 1):
+                r, t, j = _judged(em, mk(), op, f)
+                best = t if best is None else min(best, t)
+            key = f"{name}_{'fused' if fused else 'unfused'}"
+            out[key] = {"seconds": round(best, 5), "iterations": r.iterations, "qualified": j.qualified,
+                        "true_residual": j.true_residual}
+            log(f"[solve] {key}: {best * 1e3:.2f} ms, {r.iterations} it")
+    print(json.dumps(out), flush=True)
+
+
 def learned_eval(args):
     """The learned-preconditioner protocol: train the FNO on seeded random
     materials (seeds 0 .. n-1, solved at --train-grid^3), then on held-out
@@ -464,10 +493,12 @@ def main():
     ap.add_argument("--k0", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--case", default="full", choices=["full", "em-fp64", "acoustic-256", "farfield", "learned-eval"],
+    ap.add_argument("--case", default="full",
+                    choices=["full", "em-fp64", "acoustic-256", "farfield", "learned-eval", "solve"],
                     help="full: the benchmark; em-fp64 / acoustic-256: only that matvec, for profiling; "
                          "farfield: the far-field integrals, for profiling; "
-                         "learned-eval: train the FNO and compare solvers on held-out materials")
+                         "learned-eval: train the FNO and compare solvers on held-out materials; "
+                         "solve: fused and unfused solve times, for profiling")
     ap.add_argument("--train-materials", type=int, default=2000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-grid", type=int, default=64)
@@ -488,6 +519,8 @@ def main():
         return learned_eval(args)
     if args.case == "farfield":
         return farfield_case(args)
+    if args.case == "solve":
+        return solve_case(args)
     if args.case != "full":
         return profile_case(args)
     if args.gpus != 1:
