@@ -428,6 +428,80 @@ def solve_case(args):
     print(json.dumps(out), flush=True)
 
 
+def pc_ceiling(args):
+    """What a coarse-grid preconditioner -- the class the learned FNO belongs
+    to (a correction computed on a grid^3 coarse grid and interpolated back)
+    -- can save at best: right-preconditioned BiCGStab iterations to rtol 1e-8
+    at N^3 (config-2 ellipsoid, FP64) for each k0 with
+      identity;  the exact two-grid correction M = I + P (A_c^-1 - I) R
+      (A_c: the operator on the coarse grid with the averaged contrast, solved
+      to 1e-8 at every application; R: averaging, P: trilinear);  the
+      pointwise scaling (I + chi/3)^-1 (the cell's self block of A);  both.
+    Host-driven iterations (library matvecs, torch vector updates); one JSON line."""
+    import torch
+    import torch.nn.functional as F
+
+    import em3d_amd as em
+    N, fac = args.n, args.n // 32
+    be = em.Backend.hip(em.Precision.DOUBLE)
+    out = {"case": "pc-ceiling", "grid": N, "coarse_grid": N // fac}
+    nrm = lambda a: float(torch.linalg.vector_norm(a))
+    vdot = lambda a, b: complex(torch.vdot(a.flatten(), b.flatten()))
+    for k0 in args.k0s:
+        prob, _ = make_problem(em, be, N, k0)
+        op, f, chi = em.Operator(prob), prob.wave, prob.eps_tensor
+        Nc = N // fac
+        R = lambda x: x.reshape(3, Nc, fac, Nc, fac, Nc, fac).mean(dim=(2, 4, 6))
+        gc = em.Grid(N=(Nc,) * 3, L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+        chic = chi.reshape(3, 3, Nc, fac, Nc, fac, Nc, fac).mean(dim=(3, 5, 7)).contiguous()
+        opc = em.Operator(em.Problem(grid=gc, eps_tensor=chic, wave=R(f).contiguous(), k0=k0, volume=1.0))
+
+        def P(xc):
+            parts = [F.interpolate(t[None], scale_factor=fac, mode="trilinear", align_corners=False)[0]
+                     for t in (xc.real.contiguous(), xc.imag.contiguous())]
+            return torch.complex(parts[0], parts[1])
+
+        def two_grid(r):
+            rc = R(r).contiguous()
+            xc = em.BiCGStab(em.SolverConfig(max_iter=500, rtol=1e-8)).solve(opc, rc).u
+            return r + P(xc - rc)
+
+        d = torch.stack([chi[a, a] for a in range(3)])
+        jac = lambda r: r / (1.0 + d / 3.0)
+
+        def bicgstab_right(M, rtol=1e-8, maxit=300):
+            x = torch.zeros_like(f); r = f.clone(); rh = r.clone(); fn = nrm(f)
+            rho = alpha = om = 1.0
+            v = torch.zeros_like(f); p = torch.zeros_like(f)
+            for it in range(1, maxit + 1):
+                rho1 = vdot(rh, r); beta = (rho1 / rho) * (alpha / om); rho = rho1
+                p = r + beta * (p - om * v)
+                ph = M(p); v = op.matvec(ph)
+                alpha = rho / vdot(rh, v)
+                s = r - alpha * v
+                if nrm(s) / fn < rtol:
+                    return x + alpha * ph, it
+                sh = M(s); t = op.matvec(sh)
+                om = vdot(t, s) / vdot(t, t)
+                x = x + alpha * ph + om * sh
+                r = s - om * t
+                if nrm(r) / fn < rtol:
+                    return x, it
+            return x, maxit
+
+        rec = {}
+        for name, M in (("identity", lambda r: r), ("two_grid_exact", two_grid), ("pointwise_self_block", jac),
+                        ("both", lambda r: jac(two_grid(r)))):
+            x, it = bicgstab_right(M)
+            rec[name] = {"iterations": it, "true_residual": nrm(op.matvec(x) - f) / nrm(f),
+                         "one_step_residual": nrm(op.matvec(M(f)) - f) / nrm(f)}
+            log(f"[pc-ceiling] k0={k0:g} {name}: {it} it, |A M f - f|/|f| {rec[name]['one_step_residual']:.3f}")
+        out[f"k0={k0:g}"] = rec
+        del op, opc, prob
+        torch.cuda.empty_cache()
+    print(json.dumps(out), flush=True)
+
+
 def learned_eval(args):
     """The learned-preconditioner protocol: train the FNO on seeded random
     materials (seeds 0 .. n-1, solved at --train-grid^3), then on held-out
@@ -494,11 +568,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--case", default="full",
-                    choices=["full", "em-fp64", "acoustic-256", "farfield", "learned-eval", "solve"],
+                    choices=["full", "em-fp64", "acoustic-256", "farfield", "learned-eval", "solve", "pc-ceiling"],
                     help="full: the benchmark; em-fp64 / acoustic-256: only that matvec, for profiling; "
                          "farfield: the far-field integrals, for profiling; "
                          "learned-eval: train the FNO and compare solvers on held-out materials; "
-                         "solve: fused and unfused solve times, for profiling")
+                         "solve: fused and unfused solve times, for profiling; "
+                         "pc-ceiling: iterations an exact coarse-grid correction saves")
     ap.add_argument("--train-materials", type=int, default=2000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-grid", type=int, default=64)
@@ -521,6 +596,8 @@ def main():
         return farfield_case(args)
     if args.case == "solve":
         return solve_case(args)
+    if args.case == "pc-ceiling":
+        return pc_ceiling(args)
     if args.case != "full":
         return profile_case(args)
     if args.gpus != 1:
