@@ -320,14 +320,21 @@ __global__ __launch_bounds__(kT) void k_norm2(long long n, const cpx<T>* r, doub
   if (threadIdx.x == 0) part[(size_t)b * gridDim.x + blockIdx.x] = sh[0];
 }
 
-// scale[b] = sqrt(sum / n) (0 -> 1)
-__global__ void k_scale(int B, int nb, long long n, const double* part, double* scale, const int* skip) {
+// scale[b] = sqrt(sum / n) (0 -> 1): one block per right-hand side sums its
+// nb <= kT partials in a fixed tree order
+__global__ __launch_bounds__(kT) void k_scale(int nb, long long n, const double* part, double* scale,
+                                              const int* skip) {
   if (skip && *skip) return;
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  double a = 0.0;
-  for (int k = 0; k < nb; ++k) a += part[(size_t)b * nb + k];
-  const double s = sqrt(a / (double)n);
+  const int b = blockIdx.x;
+  __shared__ double sh[kT];
+  sh[threadIdx.x] = (int)threadIdx.x < nb ? part[(size_t)b * nb + threadIdx.x] : 0.0;
+  __syncthreads();
+  for (int o = kT / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x) return;
+  const double s = sqrt(sh[0] / (double)n);
   scale[b] = s > 0.0 && isfinite(s) ? s : 1.0;
 }
 
@@ -718,7 +725,7 @@ int fno_precond_apply(const em3d_fno_model* m, int B, int C, int nx, int ny, int
     hipLaunchKernelGGL(k_norm2<double>, dim3(nb, B), dim3(kT), 0, st, n, (const cpx<double>*)r, part, skip);
   else
     hipLaunchKernelGGL(k_norm2<float>, dim3(nb, B), dim3(kT), 0, st, n, (const cpx<float>*)r, part, skip);
-  hipLaunchKernelGGL(k_scale, dim3((B + 63) / 64), dim3(64), 0, st, B, (int)nb, n, part, scale, skip);
+  hipLaunchKernelGGL(k_scale, dim3(B), dim3(kT), 0, st, (int)nb, n, part, scale, skip);
   const long long cells = L.cells;
   float* feat = (float*)m->features;
   const int f = m->factor;
@@ -851,8 +858,8 @@ int em3d_fno_scale(int batch, long long n, int precision, const void* field, dou
   else
     hipLaunchKernelGGL(k_norm2<float>, dim3(nb, batch), dim3(kT), 0, stream, n, (const cpx<float>*)field,
                        (double*)work, (const int*)nullptr);
-  hipLaunchKernelGGL(k_scale, dim3((batch + 63) / 64), dim3(64), 0, stream, batch, (int)nb, n, (const double*)work,
-                     scale, (const int*)nullptr);
+  hipLaunchKernelGGL(k_scale, dim3(batch), dim3(kT), 0, stream, (int)nb, n, (const double*)work, scale,
+                     (const int*)nullptr);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : set_error(std::string("em3d_fno_scale: ") + hipGetErrorString(e));
 }

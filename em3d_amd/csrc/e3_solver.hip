@@ -429,12 +429,14 @@ __global__ __launch_bounds__(kRT) void k_group(const BiState* sts, int nb, Group
 // Start state of every RHS from the norm partials of f (r = f - A 0 = f, A 0
 // counted as one action) and its first history entry.  One thread per RHS
 // sums the nb partials in order, as a host loop would.
-__global__ void k_init(const double* part, int nb, int B, em3d_bicgstab_params prm, BiState* sts, Hist H,
-                       KryCoef* kcs) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  double nn = 0.0;
-  for (int k = 0; k < nb; ++k) nn += part[((size_t)b * kRB + k) * 4];
+// one block per right-hand side: ||f_b||^2 from the norm kernel's partials
+// (the same fixed-order block sum as the decide kernels), then the start state
+__global__ __launch_bounds__(kRT) void k_init(PartSrc ps, em3d_bicgstab_params prm, BiState* sts, Hist H,
+                                              KryCoef* kcs) {
+  const int b = blockIdx.x;
+  double nn;
+  final_sum4<1>(ps, &nn);
+  if (threadIdx.x) return;
   const double rn = sqrt(nn);
   BiState s = {};
   s.rhs_norm = rn;
@@ -696,7 +698,7 @@ int bicgstab_impl(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
       (e = hipMemcpyAsync(w.rh, rhs, vb, hipMemcpyDeviceToDevice, stream)) != hipSuccess ||
       (e = hipMemsetAsync(w.p, 0, vb, stream)) != hipSuccess || (e = hipMemsetAsync(w.v, 0, vb, stream)) != hipSuccess)
     return chk(e, "init");
-  hipLaunchKernelGGL(k_init, dim3((B + 63) / 64), dim3(64), 0, stream, (const double*)w.part, nb, B, *prm, w.st, H,
+  hipLaunchKernelGGL(k_init, dim3(B), dim3(kRT), 0, stream, vec_parts(w.part, nb), *prm, w.st, H,
                      fused ? w.kc : (KryCoef*)nullptr);
   if ((e = hipGetLastError()) != hipSuccess) return chk(e, "init launch");
 
