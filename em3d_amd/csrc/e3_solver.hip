@@ -490,6 +490,25 @@ long long field_elems(const em3d_plan_s* plan) {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Iterations to enqueue before the next state readback: for every running
+// right-hand side, ceil(log(tol / rel) / log(rate)) with rate = rel^(1/it) its
+// mean reduction per iteration (rel = 1 at it = 0); the largest of them,
+// clamped to [1, chunk].  A right-hand side with no reduction yet asks for
+// a full chunk.  The choice only moves the host checks: the iterations and
+// their results are the same for any chunk length.
+int next_chunk(const std::vector<BiState>& cur, int chunk) {
+  int len = 1;
+  for (const BiState& s : cur) {
+    if (s.done) continue;
+    const double rel = s.last_rel;
+    if (!(s.it > 0 && rel > 0.0 && rel < 1.0 && s.tol > 0.0 && std::isfinite(rel))) return chunk;
+    const double need = std::ceil(std::log(s.tol / rel) / (std::log(rel) / (double)s.it));
+    if (!(need < (double)chunk)) return chunk;
+    len = std::max(len, (int)need);
+  }
+  return len;
+}
+
 // solver scratch: r, r_hat, p, v, s, t (batch fields each) | partials | states | group | rel0 (unused)
 // | fused-step coefficients | z-pass partials of P1 and P5 | their block counts
 // | preconditioned p and s (preconditioned solves only)
@@ -703,12 +722,18 @@ int bicgstab_impl(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
   if ((e = hipGetLastError()) != hipSuccess) return chk(e, "init launch");
 
   // chunks of iterations, the states read back after each (a solve that is
-  // already finished at the start runs one chunk of no-op launches)
+  // already finished at the start runs one chunk of no-op launches).  After
+  // the first chunk, the next one is as long as the iterations the running
+  // right-hand sides still need at their mean residual reduction per
+  // iteration so far (at most `chunk`): iterations enqueued after a solve
+  // has finished still launch every kernel, each of which exits at once.
   const int chunk = prm->chunk > 0 ? prm->chunk : 4;
   std::vector<BiState> cur((size_t)B);
   bool read = false;
-  for (int it0 = 0; it0 < prm->max_iter; it0 += chunk) {
-    for (int k = 0; k < chunk && it0 + k < prm->max_iter; ++k) {
+  int len = chunk;
+  for (int it0 = 0; it0 < prm->max_iter; it0 += len) {
+    if (read) len = next_chunk(cur, chunk);
+    for (int k = 0; k < len && it0 + k < prm->max_iter; ++k) {
       int fail = 0;
       if (fused && prec == PREC_DOUBLE)
         enqueue_iteration_fused<double>(plan, khat, chi, chi_kind, u, work, w, n, nb, B, H, stream, &fail);

@@ -402,7 +402,8 @@ class _FieldApplier:
             _lib.check(lib.em3d_bicgstab_pc(*args, C.byref(precond.model(batch)),
                                             _lib.stream_handle(be.torch_device)), "em3d_bicgstab_pc")
         n_max = max(int(res[b].history_len) for b in range(batch))
-        hb = hbuf[:, :, :max(n_max, 1)].cpu()
+        # one plain copy of a small buffer; a slice of the last axis would add a gather kernel
+        hb = hbuf.cpu() if hbuf.numel() <= (1 << 19) else hbuf[:, :, :max(n_max, 1)].cpu()
         hist_h, acts_h = hb[0], hb[1].contiguous().view(torch.int64)
         out = []
         for b in range(batch):
