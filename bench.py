@@ -93,6 +93,39 @@ def train_preconditioner(em, n_materials, steps, n_train):
     return fno, info
 
 
+def support_solves(em, op, f, cfg, t_plain, reps=1):
+    """The same solve restricted to the contrast's support box
+    (em3d_amd.SupportRestricted): complex64/FP64 mixed and plain FP64
+    BiCGStab on the box, the whole field formed by one FP64 matvec of the
+    full operator, judged on the full operator.  The box operators (FP64
+    spectrum and its complex64 companion) are one-time setup per material,
+    timed apart; best of ``reps`` timed runs after one warm-up."""
+    from em3d_amd.support import support_operator
+    sub, t_setup = _timed(lambda: support_operator(op))
+    if sub is None:
+        return {"support": None}
+    _, t_comp = _timed(sub.single_precision_companion)
+    rec = {"support": {"box": list(sub.box.n), "offset": list(sub.box.lo), "fraction": round(sub.box.fraction, 4),
+                       "setup_seconds": round(t_setup + t_comp, 4)}}
+    for name, mk in (("support_mixed_bicgstab", lambda: em.SupportRestricted(em.MixedBiCGStab(cfg))),
+                     ("support_fp64_bicgstab", lambda: em.SupportRestricted(em.BiCGStab(cfg)))):
+        _judged(em, mk(), op, f)                                        # warm-up: box plans, workspaces
+        best = None
+        for _ in range(max(1, reps)):
+            r, t, j = _judged(em, mk(), op, f)
+            if best is None or t < best[1]:
+                best = (r, t, j)
+        r, t, j = best
+        rec[name] = {"seconds": round(t, 5), "iterations": r.iterations, "matvecs": r.matvec_count,
+                     "status": r.status, "true_residual": j.true_residual, "qualified": j.qualified}
+        if r.extras and "fp64_matvecs" in r.extras:
+            rec[name].update(fp32_matvecs=r.extras["fp32_matvecs"], fp64_matvecs=r.extras["fp64_matvecs"])
+        rec[f"speedup_{name}_vs_fp64"] = round(t_plain / t, 3)
+        log(f"[bench]   {name}: {t * 1e3:.2f} ms ({r.iterations} it, {r.status}, true {j.true_residual:.2e}) "
+            f"x{t_plain / t:.2f} vs plain FP64")
+    return rec
+
+
 def solve_section(em, N, k0s, learned=True):
     """Time to true rtol 1e-8 at N^3 for each k0: plain complex128 BiCGStab
     vs complex64 BiCGStab with FP64 refinement.  Wall clock from solve()
@@ -128,6 +161,7 @@ def solve_section(em, N, k0s, learned=True):
             "solution_rel_diff": float(torch.linalg.vector_norm(mixed.u - plain.u) / torch.linalg.vector_norm(plain.u)),
             "speedup_mixed_vs_fp64": round(t_plain / t_mixed, 3),
         }
+        rec.update(support_solves(em, op, f, cfg, t_plain))
         if fno is not None:
             lm, t_lm, jl = _judged(em, em.MixedBiCGStab(cfg, preconditioner=fno), op, f)
             rec["learned_pc_mixed_bicgstab"] = {
@@ -425,6 +459,10 @@ def solve_case(args):
             out[key] = {"seconds": round(best, 5), "iterations": r.iterations, "qualified": j.qualified,
                         "true_residual": j.true_residual}
             log(f"[solve] {key}: {best * 1e3:.2f} ms, {r.iterations} it")
+            if name == "fp64_bicgstab" and fused:
+                t_plain = best
+    out.update(support_solves(em, op, f, em.SolverConfig(max_iter=1000, rtol=1e-8), t_plain,
+                              reps=max(1, args.steps)))
     print(json.dumps(out), flush=True)
 
 

@@ -233,6 +233,7 @@ int em3d_plan_create(int nx, int ny, int nz, int ncomp, int batch, int precision
     delete h;
     return fail("em3d_plan_create: " + why);
   }
+  small_grid_variants(h->p.N, h->p.L, h->p.C, h->p.batch, h->p.prec, h->p.variant);
   // twiddle tables exp(-2 pi i t / L) for the three axes, float then double
   size_t nf = 0;
   for (int a = 0; a < 3; ++a) nf += (size_t)h->p.L[a];

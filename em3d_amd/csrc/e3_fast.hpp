@@ -449,10 +449,14 @@ struct XR {
 #define E3_X_MULGROUP 2
 #endif
 template <typename T, int L, int C, int TC_ = 128 / (int)sizeof(cpx<T>), bool STAGE_ = true, int WAVES_ = 2,
-          class RX = XRadix<L>, bool PAIR_ = false, bool NTX = false>
+          class RX = XRadix<L>, bool PAIR_ = false, bool NTX = false, bool RSPLIT_ = false>
 struct FastX {
   using Real = T;
   static constexpr int NCOMP = C;
+  // RSPLIT (3 components, small grids): the rows ky = p and Ly - p of a pair
+  // go to two blocks instead of one after the other in one block, which
+  // doubles the wavefronts of a launch that would not fill the chip
+  static constexpr bool RSPLIT = RSPLIT_ && C == 3;
   // PAIR (scalar operator): the rows ky = p and Ly - p of a block run together
   // as the two "components" c = 0, 1 of one schedule; they share every octant
   // entry (the scalar kernel is even in ky), loaded once for both.
@@ -493,8 +497,13 @@ struct FastX {
   int pr, ky0, ky1, pos0, tile, b;
   bool two;            // both rows ky0 = p and ky1 = Ly - p are live
   cpx<T>* lds;
+  int hsel = 0;        // RSPLIT: the row of the pair this block runs
   E3_HD FastX(const XArgs& args, long long blk, int bidx, cpx<T>* l) : a(args), b(bidx), lds(l) {
     const int ntile = args.L[2] / TC;
+    if (RSPLIT) {
+      hsel = (int)(blk & 1);
+      blk >>= 1;
+    }
     pr = (int)(blk / ntile);
     tile = (int)(blk - (long long)pr * ntile);
     pos0 = tile * TC;
@@ -502,7 +511,7 @@ struct FastX {
     ky1 = args.L[1] - pr;
     two = pr > 0 && pr < args.N[1];
   }
-  static long long blocks(const XArgs& a) { return (long long)(a.N[1] + 1) * (a.L[2] / TC); }
+  static long long blocks(const XArgs& a) { return (long long)(a.N[1] + 1) * (a.L[2] / TC) * (RSPLIT ? 2 : 1); }
 
   struct Lane {
     int h, col, j, ky;
@@ -650,6 +659,10 @@ struct FastX {
   static constexpr int PPC = 2 + 2 * MID;              // phases per component and direction (one tile)
   static constexpr int NPH = PP ? (MID ? 4 * CE + 1 : 2 * CE + 1) : 2 * CE * PPC;
   E3_HD int rows() const { return PAIR ? 1 : (two ? 2 : 1); }
+  // the rows h in [row_begin, row_end) this block runs (empty: the second
+  // row of a pair 0 or Ny, which does not exist)
+  E3_HD int row_begin() const { return RSPLIT ? hsel : 0; }
+  E3_HD int row_end() const { return RSPLIT ? (hsel == 0 || two ? hsel + 1 : hsel) : rows(); }
   // field component and row of schedule component c in row iteration h, and
   // whether that row exists (a pair 0 or Ny has one row)
   E3_HD int pcomp(int c) const { return PAIR ? 0 : c; }
@@ -659,7 +672,7 @@ struct FastX {
   template <int I>
   E3_HD void phase(int tid, int h, St& s) const {
     if constexpr (I == 0) {
-      if (h == 0) stage_tables(tid);
+      if (h == row_begin()) stage_tables(tid);
     }
     if constexpr (PP) phase_pp<I>(tid, h, s);
     else phase_one<I>(tid, h, s);
@@ -1104,8 +1117,9 @@ struct FastZI {
       v[m] = ok ? ld_nt<E3_Z_NT>(&in[a.perm ? mirror_position(n, L / 2, L) : n]) : czero<T>();
     }
   }
+  int c_first = 0;     // the block's first component (it stages the twiddle table)
   E3_HD void fft_a(int c, int tid, cpx<T>* v) const {
-    if (c == 0)
+    if (c == c_first)
       for (int t = tid; t < L; t += NT) lds[P * PS + t] = ((const cpx<T>*)a.tw)[t];
     int j = tid % R2, pl = tid / R2;
     E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
@@ -1211,12 +1225,16 @@ struct FastZI {
 // Kernel variants the plan may choose per pass kind (the autotuner times
 // them): 0 is the default configuration; alternatives exist for the
 // single-precision lengths that dominate the measured workloads.
+// The x and z passes have one more variant at the last index for every
+// length and precision: the small-grid tile (small_variant, below).
 inline int fast_variant_count(int kind, int L, int prec) {
-  if (prec != 0) return 1;
-  if (kind == 1) return (L == 256) ? 4 : (L == 128 || L == 192) ? 3 : 1;   // x pass
-  if (kind == 0) return (L == 256 || L == 192) ? 3 : 1;                      // strided passes
-  return (L == 256 || L == 192) ? 2 : 1;                                      // z passes
+  if (kind == 0) return (prec == 0 && (L == 256 || L == 192)) ? 3 : 1;      // strided passes
+  int n = 1;
+  if (prec == 0 && kind == 1) n = (L == 256) ? 4 : (L == 128 || L == 192) ? 3 : 1;   // x pass
+  if (prec == 0 && kind == 2) n = (L == 256 || L == 192) ? 2 : 1;                    // z passes
+  return n + 1;
 }
+inline int small_variant(int kind, int L, int prec) { return kind == 0 ? -2 : fast_variant_count(kind, L, prec) - 1; }
 
 // strided-pass configuration of a variant: (R1, TC)
 template <typename T, int L, int V>
@@ -1269,11 +1287,12 @@ bool fast_s_len(Run& run, const SArgs& a) {
 #define E3_X3_WAVES 2
 #endif
 
-template <typename T, int L, class RX, int TC, bool NT, class Run>
+// RSPLIT: the small-grid variant, the rows of a ky pair in two blocks
+template <typename T, int L, class RX, int TC, bool NT, class Run, bool RSPLIT = false>
 bool fast_x3(Run& run, const XArgs& a) {
   // mixed-radix lengths hold more live values: one wave per SIMD keeps them
   // in registers (VGPR + AGPR) instead of scratch
-  using X3 = FastX<T, L, 3, TC, E3_X3_STAGE, is_pow2(L) ? E3_X3_WAVES : 1, RX, false, NT>;
+  using X3 = FastX<T, L, 3, TC, E3_X3_STAGE, is_pow2(L) ? E3_X3_WAVES : 1, RX, false, NT, RSPLIT>;
   if (a.L[2] % X3::TC != 0 || X3::NT > 1024) return false;
   run.template x<X3>(a);
   return true;
@@ -1296,6 +1315,8 @@ bool fast_x_len(Run& run, const XArgs& a) {
   if (a.N[0] * 2 != L || !a.perm) return false;
   if (a.C == 3) {
     constexpr int TCd = E3_X3_TC;
+    if (a.variant == small_variant(1, L, sizeof(T) == 8 ? 1 : 0))
+      return fast_x3<T, L, XRadix<L>, TCd, NT, Run, true>(run, a);
     if constexpr (std::is_same<T, float>::value && L == 256) {
       if (a.variant == 1) return fast_x3<T, L, XR<4, 4, 16>, TCd, NT>(run, a);
       if (a.variant == 2) return fast_x3<T, L, XR<8, 4, 8>, 8, NT>(run, a);
@@ -1324,35 +1345,48 @@ struct ZVar {
 template <> struct ZVar<float, 256, 1> { static constexpr int R1 = 32, P = 32; };
 template <> struct ZVar<float, 192, 1> { static constexpr int R1 = 48, P = 32; };
 
-template <typename T, int L, int V, bool NT, class Run>
-bool fast_z_var(Run& run, const ZArgs& a) {
-  using Cfg = ZVar<T, L, V>;
+// Small grids (a support box, a packed member: a few hundred thousand
+// cells): a quarter of the pencils per block, at least one wavefront, and one
+// component per block, so the launch still spreads over every CU.
+template <class Cfg, int L>
+struct ZSmall {
+  static constexpr int R1 = Cfg::R1, R2 = L / Cfg::R1;
+  static constexpr int P = (Cfg::P / 4) * R2 >= 64 ? Cfg::P / 4 : 64 / R2;
+};
+
+template <typename T, int L, class Cfg, bool SPLIT, bool NT, class Run>
+bool fast_z_cfg(Run& run, const ZArgs& a) {
   if (a.C != 1 && a.C != 3) return false;
   if (a.n_in * 2 == L && a.n_out == L && a.inv == 0 && a.epi == EPI_STORE) {
     if (NT && a.C == 1)
-      run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, true, NT, NT>>(a);
+      run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, true, NT, NT>, SPLIT>(a);
     else
-      run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, true, NT, false>>(a);
+      run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, true, NT, false>, SPLIT>(a);
     return true;
   }
   if constexpr (NT) return false;   // the non-temporal set holds the forward pass only
   if (a.n_in == L && a.n_out * 2 == L && a.inv == 1 && a.pro == PRO_COPY &&
       (a.epi == EPI_SUB || a.epi == EPI_SUB_CHIH)) {
     if (a.C == 3 && a.epi == EPI_SUB_CHIH && a.chi_kind == CHI_FULL)
-      run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 3, true>>(a);
+      run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 3, true>, SPLIT>(a);
     else if (a.C == 3)
-      run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 3, false>>(a);
+      run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 3, false>, SPLIT>(a);
     else
-      run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 1, false>>(a);
+      run.template zi<FastZI<T, L, Cfg::R1, Cfg::P, 1, false>, SPLIT>(a);
     return true;
   }
   if constexpr (sizeof(T) == 8) {
     if (a.n_in == L && a.n_out == L && a.inv == 0 && a.pro == PRO_COPY && a.epi == EPI_STORE) {
-      run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, false>>(a);
+      run.template zf<FastZF<T, L, Cfg::R1, Cfg::P, false>, SPLIT>(a);
       return true;
     }
   }
   return false;
+}
+
+template <typename T, int L, int V, bool NT, class Run>
+bool fast_z_var(Run& run, const ZArgs& a) {
+  return fast_z_cfg<T, L, ZVar<T, L, V>, false, NT>(run, a);
 }
 
 template <typename T, int L, bool NT, class Run>
@@ -1360,7 +1394,47 @@ bool fast_z_len(Run& run, const ZArgs& a) {
   if constexpr (std::is_same<T, float>::value && (L == 256 || L == 192)) {
     if (a.variant == 1) return fast_z_var<T, L, 1, NT>(run, a);
   }
+  if (a.variant == small_variant(2, L, sizeof(T) == 8 ? 1 : 0))
+    return fast_z_cfg<T, L, ZSmall<ZVar<T, L, 0>, L>, true, NT>(run, a);
   return fast_z_var<T, L, 0, NT>(run, a);
+}
+
+// Launch sizes of the default tiles, for the plan's small-grid rule: pencils
+// per z block and the register radix of the 3-component x pass per length.
+inline int fast_z_pencils(int L, int prec) {
+#define E3_ZP(LEN) \
+  case LEN: return prec ? FastCfg<double, LEN>::P : FastCfg<float, LEN>::P;
+  switch (L) {
+    E3_ZP(16) E3_ZP(32) E3_ZP(48) E3_ZP(64) E3_ZP(80) E3_ZP(96) E3_ZP(128) E3_ZP(160) E3_ZP(192) E3_ZP(256)
+    E3_ZP(320) E3_ZP(384) E3_ZP(512)
+    default: return 0;
+  }
+#undef E3_ZP
+}
+inline int fast_x_rc(int L) {
+#define E3_XC(LEN) \
+  case LEN: return XRadix<LEN>::C;
+  switch (L) {
+    E3_XC(16) E3_XC(32) E3_XC(48) E3_XC(64) E3_XC(80) E3_XC(96) E3_XC(128) E3_XC(160) E3_XC(192) E3_XC(256)
+    E3_XC(320) E3_XC(384) E3_XC(512)
+    default: return 0;
+  }
+#undef E3_XC
+}
+
+// Default variants of a new plan (variant[1] x pass, variant[2] z passes):
+// the small-grid tiles when the default tiles would launch fewer than 1024
+// z blocks (one per component counted) or 2048 x-pass wavefronts, i.e. when a
+// launch would leave CUs idle (support boxes, packed members, grids below
+// about 96^3); 0 otherwise.
+inline void small_grid_variants(const int N[3], const int L[3], int C, int batch, int prec, int variant[3]) {
+  const int zp = fast_z_pencils(L[2], prec);
+  if (zp > 0 && ((long long)N[0] * N[1] + zp - 1) / zp * batch * C < 1024) variant[2] = small_variant(2, L[2], prec);
+  const int rc = fast_x_rc(L[0]), tc = prec ? 8 : 16;   // E3_X3_TC: 128 bytes of columns
+  if (C == 3 && rc > 0 && L[2] % tc == 0) {
+    const long long waves = (long long)(N[1] + 1) * (L[2] / tc) * ((tc * (L[0] / rc) + 63) / 64) * batch;
+    if (waves < 2048) variant[1] = small_variant(1, L[0], prec);
+  }
 }
 
 #define E3_FAST_SWITCH(FN, T, RUN, ARGS)        \

@@ -38,12 +38,13 @@ __global__ __launch_bounds__(K::NT, K::MIN_WAVES_PER_EU) void k_fast_x(XArgs a, 
   const int b = (int)(bid / ncb);
   K k(a, bid - (long long)b * ncb, b, lds);
   typename K::St st;
-  const int nrow = k.rows();
-  k.h_end = nrow;
-  if constexpr (K::PREFETCH) k.load_a(0, threadIdx.x, 0, st);
+  const int h0 = k.row_begin(), h1 = k.row_end();
+  if (h0 >= h1) return;   // block-uniform: the row does not exist
+  k.h_end = h1;
+  if constexpr (K::PREFETCH) k.load_a(0, threadIdx.x, h0, st);
 #pragma unroll 1
-  for (int h = 0; h < nrow; ++h) {
-    if (h) __syncthreads();
+  for (int h = h0; h < h1; ++h) {
+    if (h != h0) __syncthreads();
     run_phases(k, h, st, std::make_integer_sequence<int, K::NPH>{});
   }
 }
@@ -86,15 +87,16 @@ __device__ __forceinline__ void kry_block_store(const ZArgs& a, KAcc& acc) {
   }
 }
 
-// KRY: the forward pass with a fused Krylov prologue (ZArgs::k)
-template <class K, bool KRY>
+// KRY: the forward pass with a fused Krylov prologue (ZArgs::k).  SPLIT (or
+// zf_split): one block per (pencil group, component).
+template <class K, bool KRY, bool SPLIT>
 __global__ __launch_bounds__(K::NT) void k_fast_zf(ZArgs a, long long nblk) {
   if (a.skip && *a.skip) return;
   static_assert(!KRY || K::NT % 64 == 0, "fused reductions take whole wavefronts");
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
   KAcc acc{};
-  if constexpr (zf_split<K>()) {
+  if constexpr (SPLIT || zf_split<K>()) {
     const long long bid = blockIdx.x / a.C;
     const int c = (int)(blockIdx.x - bid * a.C);
     const int b = (int)(bid / nblk);
@@ -120,18 +122,37 @@ __global__ __launch_bounds__(K::NT) void k_fast_zf(ZArgs a, long long nblk) {
   }
 }
 
-// KRY: the inverse pass with fused Krylov sums in its epilogue (ZArgs::k)
-template <class K, int C, bool KRY>
+// KRY: the inverse pass with fused Krylov sums in its epilogue (ZArgs::k).
+// SPLIT (per-component epilogues only): one block per (pencil group, component).
+template <class K, int C, bool KRY, bool SPLIT>
 __global__ __launch_bounds__(K::NT) void k_fast_zi(ZArgs a, long long nblk) {
   if (a.skip && *a.skip) return;
   static_assert(!KRY || (K::NT % 64 == 0 && !K::HOLD), "fused sums: whole wavefronts, per-component epilogue");
+  static_assert(!SPLIT || !K::HOLD, "a held epilogue needs every component in one block");
   using T = typename K::Real;
   __shared__ cpx<T> lds[K::LDS];
+  typename K::St st;
+  KAcc acc{};
+  if constexpr (SPLIT) {
+    const long long bid = blockIdx.x / C;
+    const int c = (int)(blockIdx.x - bid * C);
+    const int b = (int)(bid / nblk);
+    K k(a, bid - (long long)b * nblk, b, lds);
+    k.c_first = c;
+    k.stage_a(c, threadIdx.x);
+    __syncthreads();
+    k.stage_b(c, threadIdx.x, st);
+    if constexpr (KRY) {
+      k.epilogue_one_kry(c, threadIdx.x, st, acc);
+      kry_block_store(a, acc);
+    } else {
+      k.epilogue_one(c, threadIdx.x, st);
+    }
+    return;
+  }
   const long long bid = blockIdx.x;
   const int b = (int)(bid / nblk);
   K k(a, bid - (long long)b * nblk, b, lds);
-  typename K::St st;
-  KAcc acc{};
 #pragma unroll
   for (int c = 0; c < C; ++c) {
     k.stage_a(c, threadIdx.x);
@@ -161,29 +182,30 @@ struct HipFastRun {
                        a, ncb);
     err = hipGetLastError();
   }
-  template <class K>
+  template <class K, bool SPLIT = false>
   void zf(const ZArgs& a) {
     const long long nblk = (a.P + K::P - 1) / K::P;
-    const long long split = zf_split<K>() ? a.C : 1;
+    const long long split = (SPLIT || zf_split<K>()) ? a.C : 1;
     const dim3 grid((unsigned)(nblk * a.batch * split));
     if (a.k.op == KRY_P || a.k.op == KRY_S)
-      hipLaunchKernelGGL((k_fast_zf<K, true>), grid, dim3(K::NT), 0, st, a, nblk);
+      hipLaunchKernelGGL((k_fast_zf<K, true, SPLIT>), grid, dim3(K::NT), 0, st, a, nblk);
     else
-      hipLaunchKernelGGL((k_fast_zf<K, false>), grid, dim3(K::NT), 0, st, a, nblk);
+      hipLaunchKernelGGL((k_fast_zf<K, false, SPLIT>), grid, dim3(K::NT), 0, st, a, nblk);
     err = hipGetLastError();
   }
-  template <class K>
+  template <class K, bool SPLIT_ = false>
   void zi(const ZArgs& a) {
+    constexpr bool SPLIT = SPLIT_ && !K::HOLD;
     const long long nblk = (a.P + K::P - 1) / K::P;
-    const dim3 grid((unsigned)(nblk * a.batch));
+    const dim3 grid((unsigned)(nblk * a.batch * (SPLIT ? K::NCOMP : 1)));
     if constexpr (!K::HOLD) {
       if (a.k.op == KRY_DOT || a.k.op == KRY_DOT2) {
-        hipLaunchKernelGGL((k_fast_zi<K, K::NCOMP, true>), grid, dim3(K::NT), 0, st, a, nblk);
+        hipLaunchKernelGGL((k_fast_zi<K, K::NCOMP, true, SPLIT>), grid, dim3(K::NT), 0, st, a, nblk);
         err = hipGetLastError();
         return;
       }
     }
-    hipLaunchKernelGGL((k_fast_zi<K, K::NCOMP, false>), grid, dim3(K::NT), 0, st, a, nblk);
+    hipLaunchKernelGGL((k_fast_zi<K, K::NCOMP, false, SPLIT>), grid, dim3(K::NT), 0, st, a, nblk);
     err = hipGetLastError();
   }
 };

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -568,6 +569,21 @@ Work carve(const Plan& p, void* base, size_t vb) {
   return w;
 }
 
+// The flags the passes of the matvecs test: for one right-hand side the
+// flags of its own state (no group kernel is launched), for a batch the
+// group's AND over the states (k_group after each decision that moves them).
+struct Flags {
+  const int* all_done;
+  const int* all_skip_t;
+  bool group;
+};
+Flags pass_flags(const Work& w, int B) {
+  if (B == 1)
+    return {(const int*)((const char*)w.st + offsetof(BiState, done)),
+            (const int*)((const char*)w.st + offsetof(BiState, skip_t)), false};
+  return {&w.g->all_done, &w.g->all_skip_t, true};
+}
+
 size_t work_total(const Plan& p, size_t vb) {
   const int B = p.batch;
   return 6 * align256(vb) + part_bytes(B) + align256((size_t)B * sizeof(BiState)) + align256(sizeof(Group)) +
@@ -584,12 +600,13 @@ template <typename T>
 void enqueue_iteration_fused(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, void* u, void* work,
                              const Work& w, long long n, int nb, int B, Hist H, hipStream_t stream, int* fail) {
   const dim3 gv(nb, B), gd(B);
-  const int* all_done = &w.g->all_done;
-  const int* all_skip_t = &w.g->all_skip_t;
+  const Flags fl = pass_flags(w, B);
+  const int* all_done = fl.all_done;
+  const int* all_skip_t = fl.all_skip_t;
   const PartSrc z1{w.zpart1, w.zstride, 0, &w.zcount[0], 0};
   const PartSrc z5{w.zpart5, w.zstride, 0, &w.zcount[1], 0};
   hipLaunchKernelGGL(k_rho, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, w.kc);
-  hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
+  if (fl.group) hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
   ZKry kp{KRY_P, w.r, w.v, w.p, nullptr, w.kc, w.zpart1, w.zstride, &w.zcount[0]};
   ZKry kd{KRY_DOT, nullptr, nullptr, nullptr, w.rh, w.kc, w.zpart5, w.zstride, &w.zcount[1]};
   if (apply_plan(plan, khat, chi, chi_kind, w.p, w.v, work, stream, false, "em3d_bicgstab matvec", all_done, &kp,
@@ -606,7 +623,7 @@ void enqueue_iteration_fused(em3d_plan_t plan, const void* khat, const void* chi
     return;
   }
   hipLaunchKernelGGL(k_sfin, gd, dim3(kRT), 0, stream, z1, w.st, H);
-  hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
+  if (fl.group) hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
   if (apply_plan(plan, khat, chi, chi_kind, w.s, w.t, work, stream, false, "em3d_bicgstab matvec", all_skip_t, &ks,
                  &kt, all_done, PASSES_REST)) {
     *fail = 1;
@@ -623,15 +640,16 @@ void enqueue_iteration(em3d_plan_t plan, const void* khat, const void* chi, int 
                        const Work& w, long long n, int nb, int B, Hist H, const em3d_fno_model* pc,
                        hipStream_t stream, int* fail) {
   const dim3 gv(nb, B), gd(B);
-  const int* all_done = &w.g->all_done;
-  const int* all_skip_t = &w.g->all_skip_t;
+  const Flags fl = pass_flags(w, B);
+  const int* all_done = fl.all_done;
+  const int* all_skip_t = fl.all_skip_t;
   const Plan& P = plan->p;
   void* pu = pc ? w.ph : w.p;   // search directions the matvecs and the update use
   void* su = pc ? w.sh : w.s;
   hipLaunchKernelGGL(k_dot<T>, gv, dim3(kRT), 0, stream, n, (const cpx<T>*)w.rh, (const cpx<T>*)w.r, w.part, w.st, 0);
   hipLaunchKernelGGL(k_rho, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, (KryCoef*)nullptr);
   hipLaunchKernelGGL(k_p<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)w.p, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st);
-  hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
+  if (fl.group) hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
   if (pc && fno_precond_apply(pc, B, P.C, P.N[0], P.N[1], P.N[2], P.prec, w.p, w.ph, all_done, stream)) {
     *fail = 1;
     return;
@@ -645,7 +663,7 @@ void enqueue_iteration(em3d_plan_t plan, const void* khat, const void* chi, int 
   hipLaunchKernelGGL(k_s<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)w.s, (const cpx<T>*)w.r, (const cpx<T>*)w.v, w.st,
                      w.part);
   hipLaunchKernelGGL(k_sfin, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, H);
-  hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
+  if (fl.group) hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
   if (pc && fno_precond_apply(pc, B, P.C, P.N[0], P.N[1], P.N[2], P.prec, w.s, w.sh, all_skip_t, stream)) {
     *fail = 1;
     return;

@@ -118,16 +118,18 @@ def octant_entries(khat_flat, N, ncomp: int):
     return khat_flat[torch.as_tensor(idx, device=khat_flat.device)]
 
 
-def build_octant_spectrum(N, L, k0: float, ncomp: int, precision: Precision, device_index: int = 0):
+def build_octant_spectrum(N, L, k0: float, ncomp: int, precision: Precision, device_index: int = 0, h=None):
     """Device octant spectrum in the library's storage order (1-D), FP64 build,
-    plan precision, pre-scaled by 1/(8 Nx Ny Nz)."""
+    plan precision, pre-scaled by 1/(8 Nx Ny Nz).  ``h``: the cell sizes,
+    when given (a sub-grid that must keep its parent's cell size bit for
+    bit); else L / N."""
     import torch
     plan = HipPlan.get(N, ncomp, 1, precision, device_index)
     dev = torch.device("cuda", device_index)
     es = 8 if precision is Precision.SINGLE else 16
     khat = torch.zeros(plan.khat_bytes // es, dtype=precision.torch_complex_dtype, device=dev)
     scratch = torch.empty(plan.build_bytes, dtype=torch.uint8, device=dev)
-    h = [L[a] / N[a] for a in range(3)]
+    h = [L[a] / N[a] for a in range(3)] if h is None else [float(v) for v in h]
     _lib.check(_lib.lib().em3d_kernel_build(plan.handle, float(k0), h[0], h[1], h[2], khat.data_ptr(),
                                             scratch.data_ptr(), _lib.stream_handle(dev)), "em3d_kernel_build")
     del scratch
@@ -320,7 +322,7 @@ class _FieldApplier:
                 from .backend import Backend
                 be32 = Backend.hip(Precision.SINGLE, self._be.device_index)
                 khat = build_octant_spectrum(self._N, self._L, self._k0, self._ncomp, Precision.SINGLE,
-                                             self._be.device_index)
+                                             self._be.device_index, h=getattr(self, "_h", None))
                 chi32 = None if chi is None else chi.to(torch.complex64).contiguous()
                 self._companion = _SingleCompanion(be32, self._N, self._ncomp, self._L, self._k0, khat, kind, chi32)
         return self._companion
@@ -365,11 +367,14 @@ class _FieldApplier:
             raise ValueError(f"field shape {tuple(t.shape)} != expected {fs} (or (batch,) + {fs})")
         return t.contiguous(), batch
 
-    def _device_bicgstab(self, rhs, cfg, khat, chi_kind, chi, batched: bool = False, precond=None):
+    def _device_bicgstab(self, rhs, cfg, khat, chi_kind, chi, batched: bool = False, precond=None,
+                         history: bool = True):
         """Device-resident BiCGStab of libem3d_hip on this operator.  ``batched``:
         rhs carries a leading batch axis and a list of results (one per RHS) is
         returned; every RHS runs its own recurrence and status.  ``precond``: an
-        FNOPreconditioner attached to this operator (right preconditioning)."""
+        FNOPreconditioner attached to this operator (right preconditioning).
+        ``history=False`` skips reading the residual history back (one device
+        synchronisation fewer; the result then holds the last entry only)."""
         import torch
 
         from .solvers.base import SolverResult
@@ -402,20 +407,24 @@ class _FieldApplier:
             _lib.check(lib.em3d_bicgstab_pc(*args, C.byref(precond.model(batch)),
                                             _lib.stream_handle(be.torch_device)), "em3d_bicgstab_pc")
         n_max = max(int(res[b].history_len) for b in range(batch))
-        # one plain copy of a small buffer; a slice of the last axis would add a gather kernel
-        hb = hbuf.cpu() if hbuf.numel() <= (1 << 19) else hbuf[:, :, :max(n_max, 1)].cpu()
-        hist_h, acts_h = hb[0], hb[1].contiguous().view(torch.int64)
+        if history:
+            # one plain copy of a small buffer; a slice of the last axis would add a gather kernel
+            hb = hbuf.cpu() if hbuf.numel() <= (1 << 19) else hbuf[:, :, :max(n_max, 1)].cpu()
+            hist_h, acts_h = hb[0], hb[1].contiguous().view(torch.int64)
         out = []
         for b in range(batch):
             n = int(res[b].history_len)
-            history = [float(x) for x in hist_h[b, :n].tolist()]
-            counts = [int(x) for x in acts_h[b, :n].tolist()]
+            if history:
+                hist = [float(x) for x in hist_h[b, :n].tolist()]
+                counts = [int(x) for x in acts_h[b, :n].tolist()]
+            else:
+                hist, counts = [float(res[b].true_final_residual)], [int(res[b].matvec_count)]
             status = _lib.STATUS[int(res[b].status)]
             out.append(SolverResult(u=u[b] if batched else u, iterations=int(res[b].iterations),
-                                    residual_history=history, converged=status == "converged",
+                                    residual_history=hist, converged=status == "converged",
                                     matvec_count=int(res[b].matvec_count), rmatvec_count=0,
                                     residual_action_counts=counts, status=status,
-                                    true_final_residual=float(history[-1]) if history else float("inf")))
+                                    true_final_residual=float(hist[-1]) if hist else float("inf")))
         return out if batched else out[0]
 
     def autotune(self, reps: int = 5, batch: int = 1):
@@ -472,8 +481,8 @@ class _SingleCompanion(_FieldApplier):
     def rmatvec(self, field):
         return self._apply(field, self._khat, self._chi_kind, self._chi, True)
 
-    def device_bicgstab(self, rhs, cfg, batched: bool = False, precond=None):
-        return self._device_bicgstab(rhs, cfg, self._khat, self._chi_kind, self._chi, batched, precond)
+    def device_bicgstab(self, rhs, cfg, batched: bool = False, precond=None, history: bool = True):
+        return self._device_bicgstab(rhs, cfg, self._khat, self._chi_kind, self._chi, batched, precond, history)
 
 
 class Operator(_FieldApplier):

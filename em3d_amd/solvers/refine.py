@@ -108,7 +108,7 @@ class MixedBiCGStab:
             icfg = SolverConfig(max_iter=budget if pc is None else min(budget, self.preconditioned_max_iter),
                                 rtol=min(want, 0.5), chunk=cfg.chunk, fused=getattr(cfg, "fused", True))
             saved = (x.clone(), r.clone()) if pc is not None else None   # to undo an unhelpful partial step
-            ires = comp.device_bicgstab(r32, icfg, precond=pc)
+            ires = comp.device_bicgstab(r32, icfg, precond=pc, history=False)   # outer history only
             inner_it += ires.iterations
             mv32 += ires.matvec_count
             pc_failed = pc is not None and ires.status != "converged"

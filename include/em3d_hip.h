@@ -194,6 +194,31 @@ int em3d_axpbypcz(int precision, long long n, double a_re, double a_im, const vo
 int em3d_convert_axpy(long long n, int src_precision, const void* src, double a_re, double a_im, int dst_precision,
                       void* dst, int accumulate, hipStream_t stream);
 
+/* ---- restriction to the support of the contrast --------------------------
+ * A u = u - B(chi u) (operator.py:353-363) reads u only where chi != 0, so
+ * for cells i, j of a box holding supp(chi), (A u)_i depends on u inside the
+ * box alone, through the same kernel values (offsets (i - j) h): the operator
+ * restricted to the box is the operator of a smaller grid with the same cell
+ * size.  Outside the box the solution is explicit, u = f + B(chi u).  No
+ * counterpart in the reference, which always solves on the whole grid.
+ *
+ * em3d_support_bounds: bounds (DEVICE int[6]) = x_lo, x_hi, y_lo, y_hi,
+ * z_lo, z_hi (inclusive) of the cells where any entry of the packed contrast
+ * (layouts as for em3d_matvec) is non-zero; (n, -1) on an axis when chi is
+ * zero everywhere.
+ * em3d_support_gather: dst (nplanes, bx, by, bz) = src (nplanes, nx, ny, nz)
+ * cropped at offset (x0, y0, z0); nplanes = batch x components.
+ * em3d_support_scatter: out (nplanes, nx, ny, nz) = box inside the box and,
+ * outside it, f - y when f and y are given, else 0.  out may alias f or y,
+ * not box.  The box must lie inside the grid. */
+int em3d_support_bounds(int nx, int ny, int nz, int precision, const void* chi, int chi_kind, int* bounds,
+                        hipStream_t stream);
+int em3d_support_gather(int precision, long long nplanes, int nx, int ny, int nz, int bx, int by, int bz, int x0,
+                        int y0, int z0, const void* src, void* dst, hipStream_t stream);
+int em3d_support_scatter(int precision, long long nplanes, int nx, int ny, int nz, int bx, int by, int bz, int x0,
+                         int y0, int z0, const void* box, const void* f, const void* y, void* out,
+                         hipStream_t stream);
+
 /* Far-field scatter integral F_b(d_m) = dv sum_q (chi u_b)(r_q)
  * exp(-i k0 d_m . r_q) (scatter_integral_direct, farfield/_core.py:8-71;
  * acoustics/farfield.py for ncomp = 1), accumulated in FP64.  u: (batch,

@@ -76,10 +76,12 @@ struct EmuFastRun {
       const int b = (int)(bid / ncb);
       K k(a, bid - (long long)b * ncb, b, (cpx<T>*)mem.data());
       std::vector<typename K::St> st(K::NT);
-      k.h_end = k.rows();
+      const int h0 = k.row_begin(), h1 = k.row_end();
+      if (h0 >= h1) continue;
+      k.h_end = h1;
       if constexpr (K::PREFETCH)
-        for (int t = 0; t < K::NT; ++t) k.load_a(0, t, 0, st[t]);
-      for (int h = 0; h < k.rows(); ++h) phases(k, h, st, std::make_integer_sequence<int, K::NPH>{});
+        for (int t = 0; t < K::NT; ++t) k.load_a(0, t, h0, st[t]);
+      for (int h = h0; h < h1; ++h) phases(k, h, st, std::make_integer_sequence<int, K::NPH>{});
     }
   }
   template <class K, int... I>
@@ -89,31 +91,59 @@ struct EmuFastRun {
     };
     (one(std::integral_constant<int, I>{}), ...);
   }
-  template <class K>
+  // SPLIT: one block per (pencil group, component), as the GPU launch
+  template <class K, bool SPLIT = false>
   void zf(const ZArgs& a) {
     using T = typename K::Real;
     ++g_fast_calls[2];
     const long long nblk = (a.P + K::P - 1) / K::P;
     const bool kry = a.k.op == KRY_P || a.k.op == KRY_S;
-    for (long long bid = 0; bid < nblk * a.batch; ++bid) {
+    const int per = SPLIT ? a.C : 1;
+    for (long long bid = 0; bid < nblk * a.batch * per; ++bid) {
       auto mem = tile<K>();
-      const int b = (int)(bid / nblk);
-      K k(a, bid - (long long)b * nblk, b, (cpx<T>*)mem.data());
+      const long long pg = bid / per;
+      const int b = (int)(pg / nblk);
+      K k(a, pg - (long long)b * nblk, b, (cpx<T>*)mem.data());
       std::vector<KAcc> acc(K::NT, KAcc{});
-      for (int c = 0; c < a.C; ++c) {
+      const int c0 = SPLIT ? (int)(bid % per) : 0, c1 = SPLIT ? c0 + 1 : a.C;
+      for (int c = c0; c < c1; ++c) {
         for (int t = 0; t < K::NT; ++t) {
           if (kry) k.stage_a(c, t, acc[t]);
           else k.stage_a(c, t);
         }
         for (int t = 0; t < K::NT; ++t) k.stage_b(c, t);
       }
-      if (a.k.op == KRY_S) kry_store(a, acc, b, bid - (long long)b * nblk, nblk);
+      if (a.k.op == KRY_S) kry_store(a, acc, b, bid - (long long)b * nblk * per, nblk * per);
     }
   }
-  template <class K>
+  template <class K, bool SPLIT_ = false>
   void zi(const ZArgs& a) {
     using T = typename K::Real;
     ++g_fast_calls[3];
+    constexpr bool SPLIT = SPLIT_ && !K::HOLD;
+    if constexpr (SPLIT) {
+      const long long nblk = (a.P + K::P - 1) / K::P;
+      const int C = K::NCOMP;
+      for (long long bid = 0; bid < nblk * a.batch * C; ++bid) {
+        auto mem = tile<K>();
+        const long long pg = bid / C;
+        const int c = (int)(bid % C);
+        const int b = (int)(pg / nblk);
+        K k(a, pg - (long long)b * nblk, b, (cpx<T>*)mem.data());
+        k.c_first = c;
+        std::vector<typename K::St> st(K::NT);
+        std::vector<KAcc> acc(K::NT, KAcc{});
+        for (int t = 0; t < K::NT; ++t) k.stage_a(c, t);
+        for (int t = 0; t < K::NT; ++t) k.stage_b(c, t, st[t]);
+        for (int t = 0; t < K::NT; ++t) {
+          if (a.k.op == KRY_DOT || a.k.op == KRY_DOT2) k.epilogue_one_kry(c, t, st[t], acc[t]);
+          else k.epilogue_one(c, t, st[t]);
+        }
+        if (a.k.op == KRY_DOT || a.k.op == KRY_DOT2)
+          kry_store(a, acc, b, bid - (long long)b * nblk * C, nblk * C);
+      }
+      return;
+    }
     const long long nblk = (a.P + K::P - 1) / K::P;
     for (long long bid = 0; bid < nblk * a.batch; ++bid) {
       auto mem = tile<K>();
@@ -237,6 +267,7 @@ void* emu_plan_create(int nx, int ny, int nz, int ncomp, int batch, int precisio
     delete h;
     return nullptr;
   }
+  small_grid_variants(h->p.N, h->p.L, h->p.C, h->p.batch, h->p.prec, h->p.variant);   // as em3d_plan_create
   size_t nf = 0;
   for (int a = 0; a < 3; ++a) nf += (size_t)h->p.L[a];
   h->twd.resize(2 * nf);

@@ -141,6 +141,13 @@ def test_fast_kernels_match_oracle(lib, N, L, k, prec, tol):
     for b in range(2):
         assert _rel(y[b], vie.em_matvec(chi, u[b], Kh)) < tol
         assert _rel(ya[b], vie.em_rmatvec(chi, u[b], Khat=Kh)) < tol
+    # these grids take the small-grid x and z tiles by default; the default tiles too
+    lib.emu_plan_set_variant(op.h, 1, 0)
+    lib.emu_plan_set_variant(op.h, 2, 0)
+    y, ya = op.matvec(u), op.rmatvec(u)
+    for b in range(2):
+        assert _rel(y[b], vie.em_matvec(chi, u[b], Kh)) < tol
+        assert _rel(ya[b], vie.em_rmatvec(chi, u[b], Khat=Kh)) < tol
     ca = _rand(N, 10) * 0.3
     ua = _rand((2,) + N, 11)
     opa = emu.EmuOperator(lib, N, L, k, ca, 1, precision=prec, batch=2)

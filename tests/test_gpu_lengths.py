@@ -68,28 +68,54 @@ THIN = [
 ]
 
 
+class _Tiles:
+    """The plan's x and z passes on their default tiles (variant 0) for the
+    block, instead of the small-grid tiles a thin grid gets by default; the
+    plan (shared by every operator of the shape) is restored after."""
+
+    def __init__(self, em, N, ncomp, be, tiles):
+        from em3d_amd.operator import HipPlan
+        self.plan = HipPlan.get(N, ncomp, 1, be.precision, be.device_index)
+        self.tiles = tiles
+
+    def __enter__(self):
+        self.old = self.plan.variants()
+        if self.tiles == "default":
+            self.plan.set_variants([self.old[0], 0, 0])
+        return self
+
+    def __exit__(self, *exc):
+        self.plan.set_variants(self.old)
+        return False
+
+
+@pytest.mark.parametrize("tiles", ["auto", "default"])
 @pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
 @pytest.mark.parametrize("N,L,k", THIN)
-def test_em_lengths_vs_oracle(em, N, L, k, prec, tol):
+def test_em_lengths_vs_oracle(em, N, L, k, prec, tol, tiles):
+    """auto: the plan's own variants (small-grid tiles on these thin grids);
+    default: the tiles large grids run."""
     from oracle import vie
-    _say(f"[em length] N={N} (doubled {tuple(2 * n for n in N)}) prec={prec}")
+    _say(f"[em length] N={N} (doubled {tuple(2 * n for n in N)}) prec={prec} tiles={tiles}")
     be = _be(em, prec)
     chi = _rand((3, 3) + N, 8) * 0.3
     op = em.Operator(_em_problem(em, be, N, L, k, chi))
     Kh = vie.kernel_hat(N, L, k)
     u = _rand((3,) + N, 42)
     ud = be.array(u.astype(be.complex_dtype))
-    assert _rel(_host(op.matvec(ud)), vie.em_matvec(chi, u, Kh)) < tol
-    assert _rel(_host(op.rmatvec(ud)), vie.em_rmatvec(chi, u, Khat=Kh)) < tol
+    with _Tiles(em, N, 3, be, tiles):
+        assert _rel(_host(op.matvec(ud)), vie.em_matvec(chi, u, Kh)) < tol
+        assert _rel(_host(op.rmatvec(ud)), vie.em_rmatvec(chi, u, Khat=Kh)) < tol
 
 
+@pytest.mark.parametrize("tiles", ["auto", "default"])
 @pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
 @pytest.mark.parametrize("N,L,k", THIN)
-def test_acoustic_lengths_vs_oracle(em, N, L, k, prec, tol):
+def test_acoustic_lengths_vs_oracle(em, N, L, k, prec, tol, tiles):
     """Scalar passes: the (256, 2, 8) case runs the 512-point x pass with
     1024-thread blocks and both rows of a ky pair in one schedule."""
     from oracle import vie
-    _say(f"[acoustic length] N={N} prec={prec}")
+    _say(f"[acoustic length] N={N} prec={prec} tiles={tiles}")
     be = _be(em, prec)
     grid = em.Grid(N=N, L=L, center=(0.0, 0.0, 0.0), backend=be)
     eta = 1.0 + _rand(N, 10) * 0.3
@@ -99,9 +125,10 @@ def test_acoustic_lengths_vs_oracle(em, N, L, k, prec, tol):
     chi = _host(prob.chi).astype(np.complex128)
     u = _rand(N, 11)
     ud = be.array(u.astype(be.complex_dtype))
-    assert _rel(_host(op.matvec(ud)), vie.acoustic_matvec(chi, u, vie.acoustic_kernel_hat(N, L, k))) < tol
-    assert _rel(_host(op.rmatvec(ud)),
-                vie.acoustic_rmatvec(chi, u, vie.acoustic_kernel_hat(N, L, k, adjoint=True))) < tol
+    with _Tiles(em, N, 1, be, tiles):
+        assert _rel(_host(op.matvec(ud)), vie.acoustic_matvec(chi, u, vie.acoustic_kernel_hat(N, L, k))) < tol
+        assert _rel(_host(op.rmatvec(ud)),
+                    vie.acoustic_rmatvec(chi, u, vie.acoustic_kernel_hat(N, L, k, adjoint=True))) < tol
 
 
 @pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])

@@ -1,0 +1,283 @@
+"""GPU tests of the restriction to the contrast's support (em3d_amd.support,
+em3d_amd.SupportRestricted).
+
+Oracles: the oracle's FFT operators (src/em3d/operator.py:353-398 and
+acoustics/operator.py:40-67, restated in oracle/vie.py) on the full grid and
+on the box grid (same k0, same cell size, contrast cropped), and the oracle's
+BiCGStab (src/em3d/solvers/bicgstab.py:12-211) on the full grid with its true
+residual (src/em3d/experiments/stationary_validation.py:172-180).  Bounds,
+crops and scatters are integer/byte work and are compared exactly.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def em():
+    import em3d_amd
+    return em3d_amd
+
+
+def _be(em, prec):
+    return em.Backend.hip(em.Precision.DOUBLE if prec == "double" else em.Precision.SINGLE)
+
+
+def _rand(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _host(t):
+    return t.detach().cpu().numpy()
+
+
+N0 = (32, 24, 20)
+
+
+def _em_problem(em, be, N, k, kind="diag", center=(0.1, -0.05, 0.0), radii=(0.2, 0.15, 0.2)):
+    grid = em.Grid(N=N, L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+    eps = {"iso": 2.5 + 0.1j,
+           "diag": np.diag([2.5 + 0.1j, 3.0 + 0.2j, 2.0]),
+           "full": np.array([[2.5 + 0.1j, 0.2, 0.0], [0.2, 3.0 + 0.2j, 0.1j], [0.0, 0.1j, 2.0]])}[kind]
+    chi = em.geometry.sample_contrast_tensor(grid, background_eps_r=1.0, feature_eps_r=eps,
+                                             geometry=em.geometry.Ellipsoid(center=center, radii=radii))
+    wave = em.flat_wave_vec(grid, k=k, orient=(0, 0, 1), amplitude=(1, 0, 0))
+    return em.Problem(grid=grid, eps_tensor=chi, wave=wave, k0=k, volume=1.0)
+
+
+def _np_bounds(mask):
+    out = []
+    for a in range(3):
+        other = tuple(b for b in range(3) if b != a)
+        idx = np.nonzero(mask.any(axis=other))[0]
+        out.append((int(idx[0]), int(idx[-1])))
+    return tuple(out)
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+@pytest.mark.parametrize("kind", ["iso", "diag", "full"])
+def test_support_bounds_exact(em, prec, kind):
+    """em3d_support_bounds equals the bounds of the non-zero cells of the
+    contrast, for every packing and for a contrast that is zero everywhere."""
+    from em3d_amd.operator import pack_contrast
+    from em3d_amd.support import support_bounds
+    _say(f"[support bounds] kind={kind} prec={prec}")
+    be = _be(em, prec)
+    prob = _em_problem(em, be, N0, 4.0, kind)
+    ck, chi = pack_contrast(prob.eps_tensor)
+    host = _host(prob.eps_tensor)
+    want = _np_bounds((host != 0).any(axis=(0, 1)))
+    assert support_bounds(N0, ck, chi, be.precision) == want
+    # one non-zero cell at a corner of the grid
+    import torch
+    z = torch.zeros_like(chi)
+    assert support_bounds(N0, ck, z, be.precision) is None
+    z[..., N0[0] - 1, 0, N0[2] - 1] = 1.0
+    assert support_bounds(N0, ck, z, be.precision) == ((N0[0] - 1, N0[0] - 1), (0, 0), (N0[2] - 1, N0[2] - 1))
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_gather_scatter_exact(em, prec):
+    """The crop and the two scatters equal NumPy slicing bit for bit."""
+    import torch
+    from em3d_amd.support import SupportBox, gather, scatter
+    _say(f"[support gather/scatter] prec={prec}")
+    be = _be(em, prec)
+    box = SupportBox(N=N0, lo=(5, 3, 2), n=(16, 8, 16), bounds=((5, 20), (3, 10), (2, 17)))
+    planes = 6
+    c = be.complex_dtype
+    full = _rand((planes,) + N0, 1).astype(c)
+    f = _rand((planes,) + N0, 2).astype(c)
+    y = _rand((planes,) + N0, 3).astype(c)
+    sl = (slice(None),) + tuple(slice(box.lo[a], box.lo[a] + box.n[a]) for a in range(3))
+    g = _host(gather(be.array(full), box, planes))
+    assert np.array_equal(g, full[sl])
+    bf = _rand((planes,) + box.n, 4).astype(c)
+    want0 = np.zeros_like(full)
+    want0[sl] = bf
+    assert np.array_equal(_host(scatter(be.array(bf), box, planes)), want0)
+    want1 = (f - y).astype(c)
+    want1[sl] = bf
+    yd = be.array(y)
+    out = scatter(be.array(bf), box, planes, be.array(f), yd, out=yd)   # in place over y
+    assert np.array_equal(_host(out), want1)
+    with pytest.raises(RuntimeError, match="inside the grid"):
+        scatter(be.array(bf), SupportBox(N=N0, lo=(20, 3, 2), n=(16, 8, 16), bounds=box.bounds), planes)
+    del torch
+
+
+@pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
+@pytest.mark.parametrize("kind", ["iso", "diag", "full"])
+def test_box_operator_is_the_full_operator_on_the_box(em, prec, tol, kind):
+    """(A u) on the box cells equals A_box (u on the box) for any u, and A_box
+    is the oracle's operator of the box grid: same k0, same cell size, the
+    contrast cropped.  Its adjoint is the oracle's adjoint of that grid."""
+    from oracle import vie
+    from em3d_amd.support import support_operator
+    k = 5.0
+    _say(f"[support operator] N={N0} kind={kind} prec={prec}")
+    be = _be(em, prec)
+    prob = _em_problem(em, be, N0, k, kind)
+    op = em.Operator(prob)
+    sub = support_operator(op)
+    assert sub is not None and sub.box.n == (16, 8, 8), sub.box
+    lo, n = sub.box.lo, sub.box.n
+    sl = tuple(slice(lo[a], lo[a] + n[a]) for a in range(3))
+    u = _rand((3,) + N0, 7)
+    ud = be.array(u.astype(be.complex_dtype))
+    full = _host(op.matvec(ud))[(slice(None),) + sl]
+    boxed = _host(sub.matvec(sub.restrict(ud)))
+    chi = _host(prob.eps_tensor).astype(np.complex128)
+    Lb = tuple(n[a] * (1.0 / N0[a]) for a in range(3))
+    Kb = vie.kernel_hat(n, Lb, k)
+    want = vie.em_matvec(chi[(slice(None), slice(None)) + sl], u[(slice(None),) + sl], Kb)
+    assert _rel(boxed, want) < tol
+    assert _rel(full, want) < tol
+    assert _rel(full, vie.em_matvec(chi, u, vie.kernel_hat(N0, (1, 1, 1), k))[(slice(None),) + sl]) < tol
+    v = u[(slice(None),) + sl]
+    got = _host(sub.rmatvec(be.array(v.astype(be.complex_dtype))))
+    assert _rel(got, vie.em_rmatvec(chi[(slice(None), slice(None)) + sl], v, Khat=Kb)) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
+def test_acoustic_box_operator(em, prec, tol):
+    from oracle import vie
+    from em3d_amd.support import support_operator
+    N, k = (32, 32, 24), 8.0
+    _say(f"[support acoustic operator] N={N} prec={prec}")
+    be = _be(em, prec)
+    grid = em.Grid(N=N, L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+    eta = em.acoustics.materials.eta_inclusions(grid, [((-0.1, 0.05, 0.0), (0.08, 0.08, 0.08), 2.0 + 0.1j),
+                                                       ((0.1, 0.0, 0.05), (0.06, 0.06, 0.06), 1.5 + 0.2j)])
+    prob = em.acoustics.make_acoustic_problem(grid, eta, k0=k)
+    op = em.acoustics.AcousticOperator(prob)
+    sub = support_operator(op)
+    assert sub is not None and sub.box.fraction < 0.1, sub.box
+    lo, n = sub.box.lo, sub.box.n
+    sl = tuple(slice(lo[a], lo[a] + n[a]) for a in range(3))
+    u = _rand(N, 8)
+    ud = be.array(u.astype(be.complex_dtype))
+    chi = _host(eta).astype(np.complex128) - 1.0
+    Lb = tuple(n[a] * (1.0 / N[a]) for a in range(3))
+    want = vie.acoustic_matvec(chi[sl], u[sl], vie.acoustic_kernel_hat(n, Lb, k))
+    assert _rel(_host(sub.matvec(sub.restrict(ud))), want) < tol
+    assert _rel(_host(op.matvec(ud))[sl], want) < tol
+
+
+def _oracle_op(chi, Kh, acoustic=False):
+    from oracle import vie
+
+    class Op:
+        def matvec(self, u):
+            return vie.acoustic_matvec(chi, u, Kh) if acoustic else vie.em_matvec(chi, u, Kh)
+    return Op()
+
+
+def test_restricted_solves_match_the_oracle_full_solve(em):
+    """SupportRestricted(BiCGStab) and SupportRestricted(MixedBiCGStab) reach
+    the requested true relative residual of the FULL FP64 operator and agree
+    with the oracle's full-grid BiCGStab solution."""
+    from oracle import solvers as osolv
+    from oracle import vie
+    N, k = N0, 6.0
+    _say(f"[support solve] N={N} k={k}")
+    be = _be(em, "double")
+    prob = _em_problem(em, be, N, k, "diag")
+    op = em.Operator(prob)
+    f = prob.wave
+    chi = _host(prob.eps_tensor)
+    oop = _oracle_op(chi, vie.kernel_hat(N, (1, 1, 1), k))
+    ref = osolv.bicgstab(oop, _host(f), osolv.Config(max_iter=400, rtol=1e-11))
+    assert ref.status == "converged"
+    for solver, rtol, sol_tol in ((em.BiCGStab(em.SolverConfig(max_iter=400, rtol=1e-10)), 1e-10, 1e-8),
+                                  (em.MixedBiCGStab(em.SolverConfig(max_iter=400, rtol=1e-8)), 1e-8, 1e-6)):
+        s = em.SupportRestricted(solver)
+        j = em.judge_solve(s, op, f)
+        res = j.result
+        _say(f"  {type(solver).__name__}: {res.status} it={res.iterations} true={j.true_residual:.3e} "
+             f"box={res.extras['support']}")
+        assert res.status == "converged" and j.qualified
+        assert j.true_residual < rtol
+        assert osolv.true_relative_residual(oop, _host(res.u), _host(f)) < rtol * (1 + 1e-6)
+        assert _rel(_host(res.u), ref.u) < sol_tol
+        assert res.residual_history[-1] < rtol
+
+
+def test_restricted_acoustic_solve(em):
+    from oracle import solvers as osolv
+    from oracle import vie
+    N, k = (32, 32, 24), 8.0
+    _say(f"[support acoustic solve] N={N}")
+    be = _be(em, "double")
+    grid = em.Grid(N=N, L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+    eta = em.acoustics.materials.eta_inclusions(grid, [((-0.1, 0.05, 0.0), (0.08, 0.08, 0.08), 2.0 + 0.1j),
+                                                       ((0.1, 0.0, 0.05), (0.06, 0.06, 0.06), 1.5 + 0.2j)])
+    prob = em.acoustics.make_acoustic_problem(grid, eta, k0=k)
+    op = em.acoustics.AcousticOperator(prob)
+    f = prob.wave
+    chi = _host(eta) - 1.0
+    oop = _oracle_op(chi, vie.acoustic_kernel_hat(N, (1, 1, 1), k), acoustic=True)
+    ref = osolv.bicgstab(oop, _host(f), osolv.Config(max_iter=400, rtol=1e-11))
+    j = em.judge_solve(em.SupportRestricted(em.BiCGStab(em.SolverConfig(max_iter=400, rtol=1e-10))), op, f)
+    assert j.qualified and j.true_residual < 1e-10
+    assert _rel(_host(j.result.u), ref.u) < 1e-8
+
+
+def test_restriction_follows_contrast_edits_and_falls_back(em):
+    """An in-place edit that moves the support re-plans the box; a contrast
+    that fills most of the grid runs the inner solver on the full operator,
+    with the inner solver's own result."""
+    from em3d_amd.support import support_operator
+    N, k = N0, 4.0
+    _say(f"[support edits] N={N}")
+    be = _be(em, "double")
+    prob = _em_problem(em, be, N, k, "full")
+    op = em.Operator(prob)
+    cfg = em.SolverConfig(max_iter=400, rtol=1e-10)
+    box0 = support_operator(op).box
+    prob.eps_tensor[0, 0, 1, 1, 1] = 0.5            # a cell far from the ellipsoid, in place
+    box1 = support_operator(op, max_fraction=1.0).box   # (32, 16, 16): more than half the grid
+    assert box1 != box0 and box1.lo[0] <= 1 and box1.lo[2] <= 1
+    j = em.judge_solve(em.SupportRestricted(em.BiCGStab(cfg), max_fraction=1.0), op, prob.wave)
+    assert j.qualified and j.true_residual < 1e-10
+    plain = em.BiCGStab(cfg).solve(op, prob.wave)
+    fb = em.SupportRestricted(em.BiCGStab(cfg), max_fraction=1e-3).solve(op, prob.wave)
+    assert "support" not in (fb.extras or {})
+    assert fb.iterations == plain.iterations and np.array_equal(_host(fb.u), _host(plain.u))
+
+
+def test_restricted_batch_equals_restricted_singles(em):
+    """solve_batch on the box: every right-hand side equals its own restricted
+    single solve (plane waves of one amplitude share the rescaled tolerance)."""
+    import torch
+    N, k = N0, 5.0
+    _say(f"[support batch] N={N}")
+    be = _be(em, "double")
+    prob = _em_problem(em, be, N, k, "diag")
+    op = em.Operator(prob)
+    grid = prob.grid
+    waves = [em.flat_wave_vec(grid, k=k, orient=o, amplitude=a)
+             for o, a in (((0, 0, 1), (1, 0, 0)), ((1, 0, 0), (0, 1, 0)), ((0, 1, 0), (0, 0, 1)))]
+    rhs = torch.stack(waves)
+    s = em.SupportRestricted(em.BiCGStab(em.SolverConfig(max_iter=400, rtol=1e-9)))
+    batch = s.solve_batch(op, rhs)
+    for b, w in enumerate(waves):
+        one = s.solve(op, w)
+        assert batch[b].iterations == one.iterations
+        assert _rel(_host(batch[b].u), _host(one.u)) < 1e-13
+        assert em.true_relative_residual(op, batch[b].u, w) < 1e-9
