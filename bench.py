@@ -126,6 +126,33 @@ def support_solves(em, op, f, cfg, t_plain, reps=1):
     return rec
 
 
+def time_to_rtol_summary(solve):
+    """Per k0: seconds to true rtol 1e-8 of each solver and its speed-up over
+    plain FP64 BiCGStab on the full grid (the reference's solver), and the
+    fastest qualified one."""
+    if not solve:
+        return None
+    out = {}
+    for key, rec in solve.items():
+        if not key.startswith("k0="):
+            continue
+        base = rec["fp64_bicgstab"]["seconds"]
+        row = {"fp64_bicgstab_s": base}
+        best = ("fp64_bicgstab", base)
+        for name in ("mixed_bicgstab", "learned_pc_mixed_bicgstab", "support_fp64_bicgstab", "support_mixed_bicgstab"):
+            r = rec.get(name)
+            if not r:
+                continue
+            row[f"{name}_s"] = r["seconds"]
+            row[f"speedup_{name}"] = round(base / r["seconds"], 3)
+            if r.get("qualified") and r["seconds"] < best[1]:
+                best = (name, r["seconds"])
+        row["fastest"] = best[0]
+        row["speedup_fastest"] = round(base / best[1], 3)
+        out[key] = row
+    return out
+
+
 def solve_section(em, N, k0s, learned=True):
     """Time to true rtol 1e-8 at N^3 for each k0: plain complex128 BiCGStab
     vs complex64 BiCGStab with FP64 refinement.  Wall clock from solve()
@@ -220,6 +247,10 @@ def batch_section(em, N, n_dirs, k0):
     seq, t_seq = _timed(lambda: [em.BiCGStab(cfg).solve(op, rhs[b].contiguous()) for b in range(B)])
     its = [r.iterations for r in batch]
     same = all(r.iterations == q.iterations and r.status == q.status for r, q in zip(batch, seq))
+    # the same batch on the sphere's support box (one batched box solve, one batched full matvec)
+    sup, t_sup = _timed(lambda: em.SupportRestricted(em.BiCGStab(cfg)).solve_batch(op, rhs))
+    sup_true = [em.true_relative_residual(op, sup[b].u, rhs[b]) for b in (0, B // 2, B - 1)]
+    sup_box = (sup[0].extras or {}).get("support")
     M = N ** 3
     per_rhs_bytes = (81 + 1 + 6 * ((N + 1) / N) ** 3) * M * 8
     ms = t_mv / 3 * 1e3
@@ -231,9 +262,14 @@ def batch_section(em, N, n_dirs, k0):
            "batched_solve_s": round(t_batch, 4), "sequential_solve_s": round(t_seq, 4),
            "solves_per_s_batched": round(B / t_batch, 2), "speedup_batched_vs_sequential": round(t_seq / t_batch, 3),
            "iterations_min_max": [min(its), max(its)],
-           "converged": sum(r.status == "converged" for r in batch), "matches_sequential": same}
+           "converged": sum(r.status == "converged" for r in batch), "matches_sequential": same,
+           "support_batched_solve_s": round(t_sup, 4), "support_box": sup_box,
+           "support_solves_per_s": round(B / t_sup, 2),
+           "support_converged": sum(r.status == "converged" for r in sup),
+           "support_true_residual_max_of_3": max(sup_true),
+           "speedup_support_batched_vs_batched": round(t_batch / t_sup, 3)}
     log(f"[bench] batch {B} RHS at {N}^3: matvec {ms:.2f} ms, solve {t_batch:.3f} s batched vs {t_seq:.3f} s "
-        f"sequential, its {min(its)}..{max(its)}")
+        f"sequential, its {min(its)}..{max(its)}; on the support box {t_sup:.3f} s (box {sup_box})")
     return out
 
 
@@ -774,6 +810,7 @@ def main():
                      "trace_summary": os.path.relpath(TRACE_SUMMARY, ROOT),
                      "launch_ms": round(per_launch[dom], 5),
                      "algorithmic_bytes": pb[dom]},
+        "time_to_rtol": time_to_rtol_summary(solve),
         "solve": solve,
         "batch": batch,
         "acoustic": acoustic,
