@@ -597,7 +597,23 @@ def learned_eval(args):
         plain, t_plain = _timed(lambda: em.BiCGStab(cfg).solve(op, f))
         mixed, t_mixed = _timed(lambda: em.MixedBiCGStab(cfg).solve(op, f))
         lm, t_lm = _timed(lambda: em.MixedBiCGStab(cfg, preconditioner=fno).solve(op, f))
+        # the support-restricted mixed solver; its box operators (spectrum, complex64 twin, plans) are
+        # per-material setup, timed apart
+        from em3d_amd.support import support_operator
+
+        def setup():
+            sub = support_operator(op)
+            if sub is not None:
+                comp = sub.single_precision_companion()
+                fb = sub.restrict(f)
+                sub.matvec(fb)
+                comp.matvec(fb.to(torch.complex64))
+            return sub
+        sub, t_sup_setup = _timed(setup)
+        sm, t_sm = _timed(lambda: em.SupportRestricted(em.MixedBiCGStab(cfg)).solve(op, f))
         rows.append({"seed": 100000 + i, "k0": round(mat.k0, 3), "inclusions": len(mat.inclusions),
+                     "support_s": t_sm, "support_setup_s": t_sup_setup, "support_true": _true_residual(op, sm.u, f),
+                     "support_fraction": sub.box.fraction if sub is not None else 1.0,
                      "fp64_s": t_plain, "fp64_it": plain.iterations, "fp64_ok": plain.status == "converged",
                      "mixed_s": t_mixed, "mixed_it": mixed.iterations,
                      "mixed_true": _true_residual(op, mixed.u, f),
@@ -606,7 +622,7 @@ def learned_eval(args):
                      and lm.extras["inner_status"][0] == "preconditioned converged"})
         if (i + 1) % 20 == 0:
             log(f"[bench] learned-eval: {i + 1} materials")
-        del op, f, plain, mixed, lm
+        del op, f, plain, mixed, lm, sm, sub
         torch.cuda.empty_cache()
 
     def dist(key):
@@ -615,6 +631,7 @@ def learned_eval(args):
                 "min": round(float(v.min()), 5), "max": round(float(v.max()), 5)}
     sp_mixed = [r["fp64_s"] / r["mixed_s"] for r in rows]
     sp_learned = [r["fp64_s"] / r["learned_s"] for r in rows]
+    sp_support = [r["fp64_s"] / r["support_s"] for r in rows]
     out = {"case": "learned-eval", "grid": args.n, "training": info,
            "held_out_materials": len(rows),
            "fp64_bicgstab_s": dist("fp64_s"), "fp64_iterations": dist("fp64_it"),
@@ -625,6 +642,12 @@ def learned_eval(args):
            "speedup_learned_pc_mixed_vs_fp64": {"median": round(float(np.median(sp_learned)), 3),
                                                 "min": round(float(np.min(sp_learned)), 3),
                                                 "max": round(float(np.max(sp_learned)), 3)},
+           "support_mixed_s": dist("support_s"), "support_setup_s": dist("support_setup_s"),
+           "support_box_fraction": dist("support_fraction"),
+           "speedup_support_mixed_vs_fp64": {"median": round(float(np.median(sp_support)), 3),
+                                             "min": round(float(np.min(sp_support)), 3),
+                                             "max": round(float(np.max(sp_support)), 3)},
+           "max_true_residual_support": max(r["support_true"] for r in rows),
            "preconditioned_inner_converged": int(sum(r["learned_pc_converged"] for r in rows)),
            "all_fp64_converged": all(r["fp64_ok"] for r in rows),
            "max_true_residual_mixed": max(r["mixed_true"] for r in rows),
