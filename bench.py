@@ -105,7 +105,8 @@ def support_solves(em, op, f, cfg, t_plain, reps=1):
     if sub is None:
         return {"support": None}
     _, t_comp = _timed(sub.single_precision_companion)
-    rec = {"support": {"box": list(sub.box.n), "offset": list(sub.box.lo), "fraction": round(sub.box.fraction, 4),
+    rec = {"support": {"box": list(sub.box.n), "offset": list(sub.box.lo), "order": list(sub.box.perm),
+                       "fraction": round(sub.box.fraction, 4),
                        "setup_seconds": round(t_setup + t_comp, 4)}}
     for name, mk in (("support_mixed_bicgstab", lambda: em.SupportRestricted(em.MixedBiCGStab(cfg))),
                      ("support_fp64_bicgstab", lambda: em.SupportRestricted(em.BiCGStab(cfg)))):

@@ -75,45 +75,87 @@ __global__ __launch_bounds__(kBoundThreads) void k_bounds(int nx, int ny, int nz
   }
 }
 
-// dst[p][i][j][k] = src[p][x0 + i][y0 + j][z0 + k] over planes p (batch x components)
+// The box's own axes: box axis a is grid axis perm[a] (the sides of the box
+// operator are n[perm[0]], n[perm[1]], n[perm[2]]); planes come in groups of
+// ncc = 1 (scalar), 3 (vector components, which follow the axes) or 9 (a
+// 3 x 3 tensor, both indices following the axes).
+struct BoxMap {
+  int N[3];      // grid
+  int n[3];      // box sides along the grid axes
+  int lo[3];     // box offset along the grid axes
+  int perm[3];   // box axis a = grid axis perm[a]
+  int ncc;
+};
+
+// grid plane of box plane q within its group
+__device__ __forceinline__ int grid_plane(const BoxMap& m, int q) {
+  if (m.ncc == 3) return m.perm[q];
+  if (m.ncc == 9) return 3 * m.perm[q / 3] + m.perm[q % 3];
+  return q;
+}
+
+// dst[p][d0][d1][d2] (box axes) = src[grid plane][lo + d on the grid axes]
 template <typename T>
-__global__ __launch_bounds__(256) void k_gather(long long nplanes, int nx, int ny, int nz, int bx, int by, int bz,
-                                                int x0, int y0, int z0, const cpx<T>* src, cpx<T>* dst) {
-  const long long mb = (long long)bx * by * bz;
+__global__ __launch_bounds__(256) void k_gather(long long nplanes, BoxMap m, const cpx<T>* src, cpx<T>* dst) {
+  const int b0 = m.n[m.perm[0]], b1 = m.n[m.perm[1]], b2 = m.n[m.perm[2]];
+  const long long mb = (long long)b0 * b1 * b2;
   const long long n = nplanes * mb;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
     const long long p = t / mb;
     long long r = t - p * mb;
-    const int k = (int)(r % bz);
-    r /= bz;
-    const int j = (int)(r % by), i = (int)(r / by);
-    dst[t] = src[((p * nx + (x0 + i)) * ny + (y0 + j)) * (long long)nz + (z0 + k)];
+    int d[3];
+    d[2] = (int)(r % b2);
+    r /= b2;
+    d[1] = (int)(r % b1);
+    d[0] = (int)(r / b1);
+    int g[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g[m.perm[a]] = m.lo[m.perm[a]] + d[a];
+    const long long grp = p / m.ncc;
+    const long long sp = grp * m.ncc + grid_plane(m, (int)(p - grp * m.ncc));
+    dst[t] = src[((sp * m.N[0] + g[0]) * m.N[1] + g[1]) * (long long)m.N[2] + g[2]];
   }
 }
 
-// out = box value inside the box; outside it f - y (f and y given) or 0.
+// out[p][grid] = the box value inside the box; outside it f - y (f and y given) or 0
 template <typename T>
-__global__ __launch_bounds__(256) void k_scatter(long long nplanes, int nx, int ny, int nz, int bx, int by, int bz,
-                                                 int x0, int y0, int z0, const cpx<T>* box, const cpx<T>* f,
+__global__ __launch_bounds__(256) void k_scatter(long long nplanes, BoxMap m, const cpx<T>* box, const cpx<T>* f,
                                                  const cpx<T>* y, cpx<T>* out) {
-  const long long m = (long long)nx * ny * nz;
-  const long long n = nplanes * m;
+  const long long mg = (long long)m.N[0] * m.N[1] * m.N[2];
+  const int b1 = m.n[m.perm[1]], b2 = m.n[m.perm[2]];
+  const long long n = nplanes * mg;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
-    const long long p = t / m;
-    long long r = t - p * m;
-    const int z = (int)(r % nz);
-    r /= nz;
-    const int yy = (int)(r % ny), x = (int)(r / ny);
-    const int i = x - x0, j = yy - y0, k = z - z0;
+    const long long p = t / mg;
+    long long r = t - p * mg;
+    int g[3];
+    g[2] = (int)(r % m.N[2]);
+    r /= m.N[2];
+    g[1] = (int)(r % m.N[1]);
+    g[0] = (int)(r / m.N[1]);
+    int i[3];
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      i[a] = g[a] - m.lo[a];
+      in = in && i[a] >= 0 && i[a] < m.n[a];
+    }
     cpx<T> v;
-    if (i >= 0 && i < bx && j >= 0 && j < by && k >= 0 && k < bz)
-      v = box[((p * bx + i) * by + j) * (long long)bz + k];
-    else if (f)
+    if (in) {
+      // the box plane whose grid plane is p's, within p's group
+      const long long grp = p / m.ncc;
+      const int gp = (int)(p - grp * m.ncc);
+      int q = gp;
+      for (int c = 0; c < m.ncc; ++c)
+        if (grid_plane(m, c) == gp) q = c;
+      const long long bp = grp * m.ncc + q;
+      v = box[((bp * m.n[m.perm[0]] + i[m.perm[0]]) * b1 + i[m.perm[1]]) * (long long)b2 + i[m.perm[2]]];
+    } else if (f) {
       v = f[t] - y[t];
-    else
+    } else {
       v = czero<T>();
+    }
     out[t] = v;
   }
 }
@@ -123,11 +165,24 @@ unsigned grid_for(long long n) {
   return (unsigned)(want < 1 ? 1 : (want > 65536 ? 65536 : want));
 }
 
-int check_box(const char* who, int nx, int ny, int nz, int bx, int by, int bz, int x0, int y0, int z0) {
-  if (nx <= 0 || ny <= 0 || nz <= 0) return sfail(std::string(who) + ": grid sizes must be positive");
-  if (bx <= 0 || by <= 0 || bz <= 0) return sfail(std::string(who) + ": box sizes must be positive");
-  if (x0 < 0 || y0 < 0 || z0 < 0 || x0 + bx > nx || y0 + by > ny || z0 + bz > nz)
-    return sfail(std::string(who) + ": box does not lie inside the grid");
+int make_map(const char* who, long long nplanes, int ncc, const int* grid, const int* box, const int* lo,
+             const int* perm, BoxMap* m) {
+  if (!grid || !box || !lo || !perm) return sfail(std::string(who) + ": null argument");
+  if (ncc != 1 && ncc != 3 && ncc != 9) return sfail(std::string(who) + ": ncc must be 1, 3 or 9");
+  if (nplanes < 0 || nplanes % ncc) return sfail(std::string(who) + ": nplanes must be a multiple of ncc");
+  bool seen[3] = {false, false, false};
+  for (int a = 0; a < 3; ++a) {
+    if (perm[a] < 0 || perm[a] > 2 || seen[perm[a]]) return sfail(std::string(who) + ": perm is not a permutation");
+    seen[perm[a]] = true;
+    if (grid[a] <= 0) return sfail(std::string(who) + ": grid sizes must be positive");
+    if (box[a] <= 0) return sfail(std::string(who) + ": box sizes must be positive");
+    if (lo[a] < 0 || lo[a] + box[a] > grid[a]) return sfail(std::string(who) + ": box does not lie inside the grid");
+    m->N[a] = grid[a];
+    m->n[a] = box[a];
+    m->lo[a] = lo[a];
+    m->perm[a] = perm[a];
+  }
+  m->ncc = ncc;
   return 0;
 }
 
@@ -161,41 +216,39 @@ int em3d_support_bounds(int nx, int ny, int nz, int precision, const void* chi, 
   return 0;
 }
 
-int em3d_support_gather(int precision, long long nplanes, int nx, int ny, int nz, int bx, int by, int bz, int x0,
-                        int y0, int z0, const void* src, void* dst, hipStream_t stream) {
+int em3d_support_gather(int precision, long long nplanes, int ncc, const int* grid, const int* box, const int* lo,
+                        const int* perm, const void* src, void* dst, hipStream_t stream) {
   if (!src || !dst) return sfail("em3d_support_gather: null argument");
-  if (nplanes < 0) return sfail("em3d_support_gather: negative plane count");
-  if (int rc = check_box("em3d_support_gather", nx, ny, nz, bx, by, bz, x0, y0, z0)) return rc;
-  const long long n = nplanes * bx * by * bz;
+  BoxMap m;
+  if (int rc = make_map("em3d_support_gather", nplanes, ncc, grid, box, lo, perm, &m)) return rc;
+  const long long n = nplanes * m.n[0] * m.n[1] * m.n[2];
   if (n == 0) return 0;
   if (precision == EM3D_DOUBLE)
-    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, nx, ny, nz, bx, by, bz,
-                       x0, y0, z0, (const cpx<double>*)src, (cpx<double>*)dst);
+    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, m,
+                       (const cpx<double>*)src, (cpx<double>*)dst);
   else
-    hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, nx, ny, nz, bx, by, bz,
-                       x0, y0, z0, (const cpx<float>*)src, (cpx<float>*)dst);
+    hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, m, (const cpx<float>*)src,
+                       (cpx<float>*)dst);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sfail(std::string("em3d_support_gather: ") + hipGetErrorString(e));
   return 0;
 }
 
-int em3d_support_scatter(int precision, long long nplanes, int nx, int ny, int nz, int bx, int by, int bz, int x0,
-                         int y0, int z0, const void* box, const void* f, const void* y, void* out,
+int em3d_support_scatter(int precision, long long nplanes, int ncc, const int* grid, const int* box, const int* lo,
+                         const int* perm, const void* boxed, const void* f, const void* y, void* out,
                          hipStream_t stream) {
-  if (!box || !out) return sfail("em3d_support_scatter: null argument");
+  if (!boxed || !out) return sfail("em3d_support_scatter: null argument");
   if ((f == nullptr) != (y == nullptr)) return sfail("em3d_support_scatter: give both f and y, or neither");
-  if (nplanes < 0) return sfail("em3d_support_scatter: negative plane count");
-  if (int rc = check_box("em3d_support_scatter", nx, ny, nz, bx, by, bz, x0, y0, z0)) return rc;
-  const long long n = nplanes * nx * ny * nz;
+  BoxMap m;
+  if (int rc = make_map("em3d_support_scatter", nplanes, ncc, grid, box, lo, perm, &m)) return rc;
+  const long long n = nplanes * m.N[0] * m.N[1] * m.N[2];
   if (n == 0) return 0;
   if (precision == EM3D_DOUBLE)
-    hipLaunchKernelGGL(k_scatter<double>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, nx, ny, nz, bx, by, bz,
-                       x0, y0, z0, (const cpx<double>*)box, (const cpx<double>*)f, (const cpx<double>*)y,
-                       (cpx<double>*)out);
+    hipLaunchKernelGGL(k_scatter<double>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, m,
+                       (const cpx<double>*)boxed, (const cpx<double>*)f, (const cpx<double>*)y, (cpx<double>*)out);
   else
-    hipLaunchKernelGGL(k_scatter<float>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, nx, ny, nz, bx, by, bz,
-                       x0, y0, z0, (const cpx<float>*)box, (const cpx<float>*)f, (const cpx<float>*)y,
-                       (cpx<float>*)out);
+    hipLaunchKernelGGL(k_scatter<float>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, m,
+                       (const cpx<float>*)boxed, (const cpx<float>*)f, (const cpx<float>*)y, (cpx<float>*)out);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sfail(std::string("em3d_support_scatter: ") + hipGetErrorString(e));
   return 0;

@@ -49,7 +49,8 @@ class SupportRestricted:
         fb = sub.restrict(f)
         nf, nfb = dots([(f, f), (fb, fb)])                    # one read-back for both norms
         fn, fbn = max(nf.real, 0.0) ** 0.5, max(nfb.real, 0.0) ** 0.5
-        info = {"box": list(sub.box.n), "offset": list(sub.box.lo), "fraction": round(sub.box.fraction, 5)}
+        info = {"box": list(sub.box.n), "offset": list(sub.box.lo), "order": list(sub.box.perm),
+                "fraction": round(sub.box.fraction, 5)}
         if fbn == 0.0:
             # u_box = 0 solves the box, and u = f + B(chi 0) = f everywhere
             return SolverResult(u=f.clone(), iterations=0, residual_history=[0.0], converged=True,
@@ -90,7 +91,8 @@ class SupportRestricted:
         ub = torch.stack([r.u for r in res])
         y = operator.matvec(sub.extend(ub))
         u = sub.extend(ub, f, y, out=y)
-        info = {"box": list(sub.box.n), "offset": list(sub.box.lo), "fraction": round(sub.box.fraction, 5)}
+        info = {"box": list(sub.box.n), "offset": list(sub.box.lo), "order": list(sub.box.perm),
+                "fraction": round(sub.box.fraction, 5)}
         out = []
         for b, r in enumerate(res):
             extras = dict(r.extras or {})

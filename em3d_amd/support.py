@@ -22,7 +22,12 @@ grid.  Nothing is approximated; the saving is the ratio of grid sizes (about
 10x for the 128^3 ellipsoid of radii (0.3, 0.2, 0.15)).
 
 The box's sides are rounded up to lengths whose doubled axis the fast FFT
-kernels cover (2^k, 3 2^k, 5 2^k) and kept inside the grid.
+kernels cover (2^k, 3 2^k, 5 2^k) and kept inside the grid.  The box grid may
+take its own axis order (``SupportBox.perm``): the operator is covariant
+under axis permutations when the field components and the contrast indices
+permute with the axes, and the x pass, the heaviest, is fastest on a
+power-of-two doubled axis (an 80 x 64 x 40 box runs its FP64 matvec in 87 us
+as 64 x 80 x 40, 115 us as given).
 """
 from __future__ import annotations
 
@@ -52,9 +57,15 @@ def box_side(extent: int, n: int) -> int:
 @dataclass(frozen=True)
 class SupportBox:
     N: tuple            # the full grid
-    lo: tuple           # first cell of the box per axis
-    n: tuple            # box side per axis
+    lo: tuple           # first cell of the box per grid axis
+    n: tuple            # box side per grid axis
     bounds: tuple       # ((lo, hi), ...) inclusive bounds of the non-zero cells
+    perm: tuple = (0, 1, 2)   # box axis a is grid axis perm[a]
+
+    @property
+    def shape(self) -> tuple:
+        """Sides of the box grid, in the box's own axis order."""
+        return tuple(self.n[p] for p in self.perm)
 
     @property
     def cells(self) -> int:
@@ -88,6 +99,21 @@ def support_bounds(N, kind: int, chi, precision: Precision):
     return ((h[0], h[1]), (h[2], h[3]), (h[4], h[5]))
 
 
+def _pow2(m: int) -> bool:
+    return m & (m - 1) == 0
+
+
+def box_order(n) -> tuple:
+    """Axis order of the box grid: a power-of-two side first when the first
+    side is not one (it becomes the x pass's axis), else the grid's order."""
+    if _pow2(n[0]):
+        return (0, 1, 2)
+    for a in (1, 2):
+        if _pow2(n[a]):
+            return (1, 0, 2) if a == 1 else (2, 1, 0)
+    return (0, 1, 2)
+
+
 def plan_box(N, bounds) -> SupportBox:
     """The box of fast sides holding ``bounds``, shifted to lie inside the grid."""
     lo, n = [], []
@@ -96,7 +122,8 @@ def plan_box(N, bounds) -> SupportBox:
         m = box_side(b1 - b0 + 1, int(N[a]))
         lo.append(min(b0, int(N[a]) - m))
         n.append(m)
-    return SupportBox(N=tuple(int(v) for v in N), lo=tuple(lo), n=tuple(n), bounds=tuple(bounds))
+    return SupportBox(N=tuple(int(v) for v in N), lo=tuple(lo), n=tuple(n), bounds=tuple(bounds),
+                      perm=box_order(n))
 
 
 # FP64-built spectra of box grids, keyed by (sides, h, k0, ncomp, precision,
@@ -119,25 +146,33 @@ def _box_spectrum(n, h, k0, ncomp, precision, device_index):
     return khat
 
 
-def gather(field, box: SupportBox, planes: int):
-    """(planes, box) crop of a (planes, grid) field (em3d_support_gather)."""
+def _ints(v):
+    return (C.c_int * 3)(*[int(x) for x in v])
+
+
+def gather(field, box: SupportBox, planes: int, ncc: int = 1):
+    """(planes, box shape) crop of a (planes, grid) field in the box's axis
+    order (em3d_support_gather); planes come in groups of ncc = 1, 3 (vector
+    components) or 9 (3 x 3 contrast), which follow the axes."""
     import torch
-    out = torch.empty((planes,) + tuple(box.n), dtype=field.dtype, device=field.device)
+    out = torch.empty((planes,) + box.shape, dtype=field.dtype, device=field.device)
     prec = 1 if field.dtype == torch.complex128 else 0
-    _lib.check(_lib.lib().em3d_support_gather(prec, planes, *box.N, *box.n, *box.lo, field.data_ptr(),
-                                               out.data_ptr(), _lib.stream_handle(field.device)),
+    _lib.check(_lib.lib().em3d_support_gather(prec, planes, ncc, _ints(box.N), _ints(box.n), _ints(box.lo),
+                                               _ints(box.perm), field.data_ptr(), out.data_ptr(),
+                                               _lib.stream_handle(field.device)),
                "em3d_support_gather")
     return out
 
 
-def scatter(box_field, box: SupportBox, planes: int, f=None, y=None, out=None):
-    """Full-grid field: ``box_field`` inside the box, ``f - y`` (or 0) outside
-    (em3d_support_scatter).  ``out`` may be ``f`` or ``y``."""
+def scatter(box_field, box: SupportBox, planes: int, ncc: int = 1, f=None, y=None, out=None):
+    """Full-grid field: ``box_field`` (box axis order) inside the box, ``f - y``
+    (or 0) outside (em3d_support_scatter).  ``out`` may be ``f`` or ``y``."""
     import torch
     if out is None:
         out = torch.empty((planes,) + tuple(box.N), dtype=box_field.dtype, device=box_field.device)
     prec = 1 if box_field.dtype == torch.complex128 else 0
-    _lib.check(_lib.lib().em3d_support_scatter(prec, planes, *box.N, *box.n, *box.lo, box_field.data_ptr(),
+    _lib.check(_lib.lib().em3d_support_scatter(prec, planes, ncc, _ints(box.N), _ints(box.n), _ints(box.lo),
+                                                _ints(box.perm), box_field.data_ptr(),
                                                 None if f is None else f.data_ptr(),
                                                 None if y is None else y.data_ptr(), out.data_ptr(),
                                                 _lib.stream_handle(box_field.device)),
@@ -153,17 +188,19 @@ class SupportOperator(_FieldApplier):
 
     def __init__(self, parent, box: SupportBox):
         be = parent.backend
-        h = tuple(parent._L[a] / parent._N[a] for a in range(3))
-        self._init_applier(be, box.n, parent._ncomp, tuple(box.n[a] * h[a] for a in range(3)), parent._k0)
+        hg = tuple(parent._L[a] / parent._N[a] for a in range(3))
+        h = tuple(hg[p] for p in box.perm)           # cell sizes along the box's axes
+        shape = box.shape
+        self._init_applier(be, shape, parent._ncomp, tuple(shape[a] * h[a] for a in range(3)), parent._k0)
         self._h = h
         self.box = box
         self.parent = parent
         _, kind, chi = parent._spectrum()
         self._parent_chi = chi
-        self._khat = _box_spectrum(box.n, h, parent._k0, parent._ncomp, be.precision, be.device_index)
+        self._khat = _box_spectrum(shape, h, parent._k0, parent._ncomp, be.precision, be.device_index)
         self._chi_kind = kind
-        self._chi = gather(chi.reshape((-1,) + tuple(box.N)), box, _chi_entries(kind)).reshape(
-            chi.shape[:-3] + tuple(box.n))
+        ne = _chi_entries(kind)
+        self._chi = gather(chi.reshape((-1,) + tuple(box.N)), box, ne, ne).reshape(chi.shape[:-3] + shape)
 
     @property
     def backend(self):
@@ -187,14 +224,15 @@ class SupportOperator(_FieldApplier):
         return lead, int(prod(lead)) if lead else 1
 
     def restrict(self, field):
-        """The box part of a full-grid field ((batch,) + field shape)."""
+        """The box part of a full-grid field ((batch,) + field shape), in the
+        box's axis order (components too)."""
         lead, planes = self._planes(field, 3)
-        return gather(field.contiguous(), self.box, planes).reshape(lead + tuple(self.box.n))
+        return gather(field.contiguous(), self.box, planes, self._ncomp).reshape(lead + self.box.shape)
 
     def extend(self, box_field, f=None, y=None, out=None):
         """Full-grid field: the box values inside, ``f - y`` (or zero) outside."""
         lead, planes = self._planes(box_field, 3)
-        res = scatter(box_field.contiguous(), self.box, planes, f, y, out)
+        res = scatter(box_field.contiguous(), self.box, planes, self._ncomp, f, y, out)
         return res.reshape(lead + tuple(self.box.N))
 
 

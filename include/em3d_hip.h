@@ -206,17 +206,23 @@ int em3d_convert_axpy(long long n, int src_precision, const void* src, double a_
  * z_lo, z_hi (inclusive) of the cells where any entry of the packed contrast
  * (layouts as for em3d_matvec) is non-zero; (n, -1) on an axis when chi is
  * zero everywhere.
- * em3d_support_gather: dst (nplanes, bx, by, bz) = src (nplanes, nx, ny, nz)
- * cropped at offset (x0, y0, z0); nplanes = batch x components.
- * em3d_support_scatter: out (nplanes, nx, ny, nz) = box inside the box and,
- * outside it, f - y when f and y are given, else 0.  out may alias f or y,
- * not box.  The box must lie inside the grid. */
+ * The box may take its own axis order: box axis a is grid axis perm[a], so
+ * the box grid has sides box[perm[0]], box[perm[1]], box[perm[2]] (the
+ * operator is covariant under axis permutations when vector components and
+ * contrast indices permute with the axes).  grid, box, lo and perm are HOST
+ * int[3] arrays in grid-axis order; planes come in groups of ncc = 1
+ * (scalar), 3 (vector components) or 9 (3 x 3 contrast entries).
+ * em3d_support_gather: dst (nplanes, box axes) = src (nplanes, nx, ny, nz)
+ * cropped at offset lo, components permuted; nplanes = batch x ncc.
+ * em3d_support_scatter: out (nplanes, nx, ny, nz) = the box field inside the
+ * box and, outside it, f - y when f and y are given, else 0.  out may alias
+ * f or y, not the box field.  The box must lie inside the grid. */
 int em3d_support_bounds(int nx, int ny, int nz, int precision, const void* chi, int chi_kind, int* bounds,
                         hipStream_t stream);
-int em3d_support_gather(int precision, long long nplanes, int nx, int ny, int nz, int bx, int by, int bz, int x0,
-                        int y0, int z0, const void* src, void* dst, hipStream_t stream);
-int em3d_support_scatter(int precision, long long nplanes, int nx, int ny, int nz, int bx, int by, int bz, int x0,
-                         int y0, int z0, const void* box, const void* f, const void* y, void* out,
+int em3d_support_gather(int precision, long long nplanes, int ncc, const int* grid, const int* box, const int* lo,
+                        const int* perm, const void* src, void* dst, hipStream_t stream);
+int em3d_support_scatter(int precision, long long nplanes, int ncc, const int* grid, const int* box, const int* lo,
+                         const int* perm, const void* boxed, const void* f, const void* y, void* out,
                          hipStream_t stream);
 
 /* Far-field scatter integral F_b(d_m) = dv sum_q (chi u_b)(r_q)

@@ -91,34 +91,50 @@ def test_support_bounds_exact(em, prec, kind):
     assert support_bounds(N0, ck, z, be.precision) == ((N0[0] - 1, N0[0] - 1), (0, 0), (N0[2] - 1, N0[2] - 1))
 
 
+def _np_box(full, box, ncc):
+    """NumPy reference of the crop: planes in groups of ncc, the box's axis
+    order, components following the axes (ncc 3) or both indices (ncc 9)."""
+    planes = full.shape[0]
+    sl = tuple(slice(box.lo[a], box.lo[a] + box.n[a]) for a in range(3))
+    out = np.empty((planes,) + box.shape, dtype=full.dtype)
+    perm = box.perm
+    for p in range(planes):
+        g, q = divmod(p, ncc)
+        src = q if ncc == 1 else (perm[q] if ncc == 3 else 3 * perm[q // 3] + perm[q % 3])
+        out[p] = np.transpose(full[g * ncc + src][sl], perm)
+    return out
+
+
 @pytest.mark.parametrize("prec", ["double", "single"])
-def test_gather_scatter_exact(em, prec):
-    """The crop and the two scatters equal NumPy slicing bit for bit."""
-    import torch
+@pytest.mark.parametrize("perm,ncc", [((0, 1, 2), 1), ((1, 0, 2), 3), ((2, 1, 0), 9), ((1, 2, 0), 3)])
+def test_gather_scatter_exact(em, prec, perm, ncc):
+    """The crop and the two scatters equal NumPy slicing and transposes bit
+    for bit, in every box axis order and plane grouping."""
     from em3d_amd.support import SupportBox, gather, scatter
-    _say(f"[support gather/scatter] prec={prec}")
+    _say(f"[support gather/scatter] prec={prec} perm={perm} ncc={ncc}")
     be = _be(em, prec)
-    box = SupportBox(N=N0, lo=(5, 3, 2), n=(16, 8, 16), bounds=((5, 20), (3, 10), (2, 17)))
-    planes = 6
+    box = SupportBox(N=N0, lo=(5, 3, 2), n=(16, 8, 16), bounds=((5, 20), (3, 10), (2, 17)), perm=perm)
+    planes = 2 * ncc
     c = be.complex_dtype
     full = _rand((planes,) + N0, 1).astype(c)
     f = _rand((planes,) + N0, 2).astype(c)
     y = _rand((planes,) + N0, 3).astype(c)
-    sl = (slice(None),) + tuple(slice(box.lo[a], box.lo[a] + box.n[a]) for a in range(3))
-    g = _host(gather(be.array(full), box, planes))
-    assert np.array_equal(g, full[sl])
-    bf = _rand((planes,) + box.n, 4).astype(c)
-    want0 = np.zeros_like(full)
-    want0[sl] = bf
-    assert np.array_equal(_host(scatter(be.array(bf), box, planes)), want0)
-    want1 = (f - y).astype(c)
-    want1[sl] = bf
+    g = _host(gather(be.array(full), box, planes, ncc))
+    assert np.array_equal(g, _np_box(full, box, ncc))
+    # scatter is the inverse of gather inside the box
+    bf = _rand((planes,) + box.shape, 4).astype(c)
+    back = _host(scatter(be.array(bf), box, planes, ncc))
+    assert np.array_equal(_np_box(back, box, ncc), bf)
+    inside = np.zeros(N0, bool)
+    inside[tuple(slice(box.lo[a], box.lo[a] + box.n[a]) for a in range(3))] = True
+    assert not np.any(back[:, ~inside])
     yd = be.array(y)
-    out = scatter(be.array(bf), box, planes, be.array(f), yd, out=yd)   # in place over y
-    assert np.array_equal(_host(out), want1)
+    out = _host(scatter(be.array(bf), box, planes, ncc, be.array(f), yd, out=yd))   # in place over y
+    assert np.array_equal(out[:, ~inside], (f - y)[:, ~inside])
+    assert np.array_equal(_np_box(out, box, ncc), bf)
     with pytest.raises(RuntimeError, match="inside the grid"):
-        scatter(be.array(bf), SupportBox(N=N0, lo=(20, 3, 2), n=(16, 8, 16), bounds=box.bounds), planes)
-    del torch
+        scatter(be.array(bf), SupportBox(N=N0, lo=(20, 3, 2), n=(16, 8, 16), bounds=box.bounds, perm=perm), planes,
+                ncc)
 
 
 @pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
@@ -135,23 +151,50 @@ def test_box_operator_is_the_full_operator_on_the_box(em, prec, tol, kind):
     prob = _em_problem(em, be, N0, k, kind)
     op = em.Operator(prob)
     sub = support_operator(op)
-    assert sub is not None and sub.box.n == (16, 8, 8), sub.box
+    assert sub is not None and sub.box.n == (16, 8, 8) and sub.box.perm == (0, 1, 2), sub.box
+    _check_box_operator(em, op, sub, prob, k, be, tol)
+
+
+def _check_box_operator(em, op, sub, prob, k, be, tol):
+    """A_box (in its own axis order, mapped back to the grid) against the
+    oracle's operator of the box grid in the grid's order, and against the
+    full operator on the box; the box adjoint against the oracle's."""
+    from oracle import vie
+    N = tuple(prob.grid.N)
     lo, n = sub.box.lo, sub.box.n
-    sl = tuple(slice(lo[a], lo[a] + n[a]) for a in range(3))
-    u = _rand((3,) + N0, 7)
+    sl = (slice(None),) + tuple(slice(lo[a], lo[a] + n[a]) for a in range(3))
+    u = _rand((3,) + N, 7)
     ud = be.array(u.astype(be.complex_dtype))
-    full = _host(op.matvec(ud))[(slice(None),) + sl]
-    boxed = _host(sub.matvec(sub.restrict(ud)))
+    full = _host(op.matvec(ud))[sl]
+    boxed = _host(sub.extend(sub.matvec(sub.restrict(ud))))[sl]
     chi = _host(prob.eps_tensor).astype(np.complex128)
-    Lb = tuple(n[a] * (1.0 / N0[a]) for a in range(3))
+    Lb = tuple(n[a] * (1.0 / N[a]) for a in range(3))
     Kb = vie.kernel_hat(n, Lb, k)
-    want = vie.em_matvec(chi[(slice(None), slice(None)) + sl], u[(slice(None),) + sl], Kb)
+    chib = chi[(slice(None),) + sl]
+    want = vie.em_matvec(chib, u[sl], Kb)
     assert _rel(boxed, want) < tol
     assert _rel(full, want) < tol
-    assert _rel(full, vie.em_matvec(chi, u, vie.kernel_hat(N0, (1, 1, 1), k))[(slice(None),) + sl]) < tol
-    v = u[(slice(None),) + sl]
-    got = _host(sub.rmatvec(be.array(v.astype(be.complex_dtype))))
-    assert _rel(got, vie.em_rmatvec(chi[(slice(None), slice(None)) + sl], v, Khat=Kb)) < tol
+    assert _rel(full, vie.em_matvec(chi, u, vie.kernel_hat(N, (1, 1, 1), k))[sl]) < tol
+    got = _host(sub.extend(sub.rmatvec(sub.restrict(ud))))[sl]
+    assert _rel(got, vie.em_rmatvec(chib, u[sl], Khat=Kb)) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
+@pytest.mark.parametrize("kind", ["diag", "full"])
+def test_permuted_box_operator(em, prec, tol, kind):
+    """A box whose x side is not a power of two takes a power-of-two side as
+    its first axis (its x pass): the permuted box operator, with components
+    and contrast entries permuted, is still the full operator on the box."""
+    from em3d_amd.support import support_operator
+    N, k = (48, 32, 24), 5.0
+    _say(f"[support permuted operator] N={N} kind={kind} prec={prec}")
+    be = _be(em, prec)
+    prob = _em_problem(em, be, N, k, kind, center=(0.05, 0.0, 0.02), radii=(0.22, 0.2, 0.15))
+    op = em.Operator(prob)
+    sub = support_operator(op)
+    assert sub is not None and sub.box.perm != (0, 1, 2), sub.box
+    assert sub.box.shape[0] & (sub.box.shape[0] - 1) == 0, sub.box
+    _check_box_operator(em, op, sub, prob, k, be, tol)
 
 
 @pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
@@ -175,7 +218,7 @@ def test_acoustic_box_operator(em, prec, tol):
     chi = _host(eta).astype(np.complex128) - 1.0
     Lb = tuple(n[a] * (1.0 / N[a]) for a in range(3))
     want = vie.acoustic_matvec(chi[sl], u[sl], vie.acoustic_kernel_hat(n, Lb, k))
-    assert _rel(_host(sub.matvec(sub.restrict(ud))), want) < tol
+    assert _rel(_host(sub.extend(sub.matvec(sub.restrict(ud))))[sl], want) < tol
     assert _rel(_host(op.matvec(ud))[sl], want) < tol
 
 

@@ -49,3 +49,17 @@ def test_plan_box_holds_the_support_inside_the_grid():
 def test_box_fraction():
     b = SupportBox(N=(128, 128, 128), lo=(24, 32, 44), n=(80, 64, 40), bounds=((25, 101), (38, 89), (45, 83)))
     assert b.cells == 204800 and abs(b.fraction - 204800 / 128 ** 3) < 1e-15 and not b.is_full
+
+
+@pytest.mark.parametrize("n,perm", [((80, 64, 40), (1, 0, 2)), ((64, 80, 40), (0, 1, 2)), ((80, 40, 32), (2, 1, 0)),
+                                    ((24, 48, 96), (0, 1, 2)), ((128, 96, 80), (0, 1, 2))])
+def test_box_order_puts_a_power_of_two_side_first(n, perm):
+    from em3d_amd.support import box_order
+    assert box_order(n) == perm
+    assert sorted(perm) == [0, 1, 2]
+
+
+def test_plan_box_order_and_shape():
+    bounds = ((25, 101), (38, 89), (45, 83))        # the 128^3 ellipsoid of radii (0.3, 0.2, 0.15)
+    box = plan_box((128, 128, 128), bounds)
+    assert box.n == (80, 64, 40) and box.perm == (1, 0, 2) and box.shape == (64, 80, 40)
