@@ -280,7 +280,22 @@ __device__ __forceinline__ void final_sum4(PartSrc ps, double* out) {
   double v[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) v[i] = 0.0;
-  for (int k = threadIdx.x; k < np; k += blockDim.x)
+  // four records per step, all loaded before the first add: the loads are in
+  // flight together, and the additions keep the record order
+  const int bd = (int)blockDim.x;
+  int k = threadIdx.x;
+  for (; k + 3 * bd < np; k += 4 * bd) {
+    double q[4][NV];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < NV; ++i) q[j][i] = part[(size_t)(k + j * bd) * 4 + i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < NV; ++i) v[i] += q[j][i];
+  }
+  for (; k < np; k += bd)
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] += part[(size_t)k * 4 + i];
   bsum<NV>(v);
@@ -314,14 +329,9 @@ __device__ __forceinline__ void record_b(BiState* st, Hist H, double v) {
   st->nhist += 1;
 }
 
-__global__ __launch_bounds__(kRT) void k_rho(PartSrc ps, BiState* sts, KryCoef* kcs) {
-  BiState* st = sts + blockIdx.x;
-  if (st->done) return;
-  double v[2];
-  final_sum4<2>(ps, v);
-  if (threadIdx.x) return;
-  Publish pub{st, kcs ? kcs + blockIdx.x : nullptr};
-  const cpx<double> rho = mk(v[0], v[1]);
+// rho = (r_hat, r) of the iteration about to start, and its beta
+__device__ __forceinline__ void rho_step(BiState* st, double re, double im) {
+  const cpx<double> rho = mk(re, im);
   st->rho = rho;
   if (cabs_d(rho) == 0.0) return finish(st, ST_BREAKDOWN_RHO);
   if (st->it > 0) {
@@ -329,6 +339,16 @@ __global__ __launch_bounds__(kRT) void k_rho(PartSrc ps, BiState* sts, KryCoef* 
     st->beta = cdiv(rho, st->rho_prev) * cdiv(st->alpha, st->omega);
   }
   if (st->use_max_act && st->mv + 1 > st->max_act) return finish(st, ST_MAX_ACTIONS);
+}
+
+__global__ __launch_bounds__(kRT) void k_rho(PartSrc ps, BiState* sts, KryCoef* kcs) {
+  BiState* st = sts + blockIdx.x;
+  if (st->done) return;
+  double v[2];
+  final_sum4<2>(ps, v);
+  if (threadIdx.x) return;
+  Publish pub{st, kcs ? kcs + blockIdx.x : nullptr};
+  rho_step(st, v[0], v[1]);
 }
 
 __global__ __launch_bounds__(kRT) void k_alpha(PartSrc ps, BiState* sts, KryCoef* kcs) {
@@ -382,6 +402,22 @@ __global__ __launch_bounds__(kRT) void k_omega(PartSrc ps, BiState* sts, Hist H)
   }
 }
 
+// end of an iteration: |r|^2 of the updated residual, history, stop tests
+__device__ __forceinline__ void rfin_step(BiState* st, Hist H, double rr) {
+  st->it += 1;
+  if (st->halt) {
+    st->done = 1;
+    return;
+  }
+  const double rel = sqrt(rr) / st->rhs_norm;
+  record_b(st, H, rel);
+  if (!isfinite(rel)) return finish(st, ST_NONFINITE);
+  if (rel < st->tol) return finish(st, ST_CONVERGED);
+  if (st->use_divg && rel > st->divg) return finish(st, ST_DIVERGENCE);
+  st->rho_prev = st->rho;
+  if (st->it >= st->max_iter) return finish(st, ST_MAX_ITER);
+}
+
 __global__ __launch_bounds__(kRT) void k_rfin(PartSrc ps, BiState* sts, Hist H, KryCoef* kcs) {
   BiState* st = sts + blockIdx.x;
   if (st->done) return;
@@ -389,18 +425,20 @@ __global__ __launch_bounds__(kRT) void k_rfin(PartSrc ps, BiState* sts, Hist H, 
   final_sum4<1>(ps, &v);
   if (threadIdx.x) return;
   Publish pub{st, kcs ? kcs + blockIdx.x : nullptr};
-  st->it += 1;
-  if (st->halt) {
-    st->done = 1;
-    return;
-  }
-  const double rel = sqrt(v) / st->rhs_norm;
-  record_b(st, H, rel);
-  if (!isfinite(rel)) return finish(st, ST_NONFINITE);
-  if (rel < st->tol) return finish(st, ST_CONVERGED);
-  if (st->use_divg && rel > st->divg) return finish(st, ST_DIVERGENCE);
-  st->rho_prev = st->rho;
-  if (st->it >= st->max_iter) return finish(st, ST_MAX_ITER);
+  rfin_step(st, H, v);
+}
+
+// k_rfin, then (the solve going on) k_rho of the next iteration, in one
+// launch: k_ur_dot's records hold (r_hat, r) and |r|^2 side by side
+__global__ __launch_bounds__(kRT) void k_rfin_rho(PartSrc ps, BiState* sts, Hist H, KryCoef* kcs) {
+  BiState* st = sts + blockIdx.x;
+  if (st->done) return;
+  double v[3];
+  final_sum4<3>(ps, v);
+  if (threadIdx.x) return;
+  Publish pub{st, kcs ? kcs + blockIdx.x : nullptr};
+  rfin_step(st, H, v[2]);
+  if (!st->done) rho_step(st, v[0], v[1]);
 }
 
 // group flags of the batched matvecs: all_done = AND done, all_skip_t = AND skip_t
@@ -435,9 +473,10 @@ __global__ __launch_bounds__(kRT) void k_group(const BiState* sts, int nb, Group
 __global__ __launch_bounds__(kRT) void k_init(PartSrc ps, em3d_bicgstab_params prm, BiState* sts, Hist H,
                                               KryCoef* kcs) {
   const int b = blockIdx.x;
-  double nn;
-  final_sum4<1>(ps, &nn);
+  double ff[2];          // (f, f): |f|^2 and its imaginary part as summed
+  final_sum4<2>(ps, ff);
   if (threadIdx.x) return;
+  const double nn = ff[0];
   const double rn = sqrt(nn);
   BiState s = {};
   s.rhs_norm = rn;
@@ -470,11 +509,17 @@ __global__ __launch_bounds__(kRT) void k_init(PartSrc ps, em3d_bicgstab_params p
   if (s.status != ST_RUNNING) {
     s.done = 1;
     s.skip_t = 1;
+  } else if (kcs) {
+    // the fused iterations start without k_rho: r_hat = r = f, so the first
+    // rho is (f, f), the sum just taken
+    rho_step(&s, ff[0], ff[1]);
   }
   sts[b] = s;
   H.h[(size_t)b * H.cap] = rel0;
   H.a[(size_t)b * H.cap] = s.mv;
-  if (kcs) Publish{&sts[b], kcs + b};
+  if (kcs) {
+    Publish pub{&sts[b], kcs + b};   // publishes as it goes out of scope
+  }
 }
 
 int sfail(const std::string& m) { return set_error(m); }
@@ -605,8 +650,6 @@ void enqueue_iteration_fused(em3d_plan_t plan, const void* khat, const void* chi
   const int* all_skip_t = fl.all_skip_t;
   const PartSrc z1{w.zpart1, w.zstride, 0, &w.zcount[0], 0};
   const PartSrc z5{w.zpart5, w.zstride, 0, &w.zcount[1], 0};
-  hipLaunchKernelGGL(k_rho, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, w.kc);
-  if (fl.group) hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
   ZKry kp{KRY_P, w.r, w.v, w.p, nullptr, w.kc, w.zpart1, w.zstride, &w.zcount[0]};
   ZKry kd{KRY_DOT, nullptr, nullptr, nullptr, w.rh, w.kc, w.zpart5, w.zstride, &w.zcount[1]};
   if (apply_plan(plan, khat, chi, chi_kind, w.p, w.v, work, stream, false, "em3d_bicgstab matvec", all_done, &kp,
@@ -632,7 +675,8 @@ void enqueue_iteration_fused(em3d_plan_t plan, const void* khat, const void* chi
   hipLaunchKernelGGL(k_omega, gd, dim3(kRT), 0, stream, z5, w.st, H);
   hipLaunchKernelGGL(k_ur_dot<T>, gv, dim3(kRT), 0, stream, n, (cpx<T>*)u, (const cpx<T>*)w.p, (const cpx<T>*)w.s,
                      (cpx<T>*)w.r, (const cpx<T>*)w.t, (const cpx<T>*)w.rh, w.st, w.part);
-  hipLaunchKernelGGL(k_rfin, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb, 2), w.st, H, w.kc);
+  hipLaunchKernelGGL(k_rfin_rho, gd, dim3(kRT), 0, stream, vec_parts(w.part, nb), w.st, H, w.kc);
+  if (fl.group) hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
 }
 
 template <typename T>
@@ -737,6 +781,7 @@ int bicgstab_impl(em3d_plan_t plan, const void* khat, const void* chi, int chi_k
     return chk(e, "init");
   hipLaunchKernelGGL(k_init, dim3(B), dim3(kRT), 0, stream, vec_parts(w.part, nb), *prm, w.st, H,
                      fused ? w.kc : (KryCoef*)nullptr);
+  if (fused && pass_flags(w, B).group) hipLaunchKernelGGL(k_group, dim3(1), dim3(kRT), 0, stream, w.st, B, w.g);
   if ((e = hipGetLastError()) != hipSuccess) return chk(e, "init launch");
 
   // chunks of iterations, the states read back after each (a solve that is

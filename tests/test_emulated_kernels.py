@@ -179,13 +179,14 @@ def test_generic_and_fast_paths_agree(lib):
     assert _rel(fast, slow) < 1e-14
 
 
-VARIANT_CASES = [((128, 2, 8), 1), ((96, 2, 8), 1), ((64, 2, 8), 1),     # x pass at 256, 192, 128
-                 ((2, 128, 8), 0), ((2, 96, 8), 0),                      # strided passes at 256, 192
-                 ((2, 2, 128), 2), ((2, 2, 96), 2)]                      # z passes at 256, 192
+VARIANT_CASES = [((128, 2, 8), 1, "single"), ((96, 2, 8), 1, "single"), ((64, 2, 8), 1, "single"),  # x: 256/192/128
+                 ((2, 128, 8), 0, "single"), ((2, 96, 8), 0, "single"),          # strided passes at 256, 192
+                 ((2, 2, 128), 2, "single"), ((2, 2, 96), 2, "single"),          # z passes at 256, 192
+                 ((128, 2, 8), 1, "double")]                                     # FP64 x pass at 256
 
 
-@pytest.mark.parametrize("N,kind", VARIANT_CASES)
-def test_every_kernel_variant_matches_oracle(lib, N, kind):
+@pytest.mark.parametrize("N,kind,precision", VARIANT_CASES)
+def test_every_kernel_variant_matches_oracle(lib, N, kind, precision):
     """Each variant the autotuner may pick (and the generic kernels, -1)
     gives the oracle's matvec and rmatvec."""
     L, k = (1.0, 0.6, 0.8), 3.0
@@ -193,10 +194,11 @@ def test_every_kernel_variant_matches_oracle(lib, N, kind):
     u = _rand((1, 3) + N, 22)
     Kh = vie.kernel_hat(N, L, k)
     ref, refa = vie.em_matvec(chi, u[0], Kh), vie.em_rmatvec(chi, u[0], Khat=Kh)
-    op = emu.EmuOperator(lib, N, L, k, chi, 3, precision="single", batch=1)
+    op = emu.EmuOperator(lib, N, L, k, chi, 3, precision=precision, batch=1)
+    tol = 1e-6 if precision == "single" else 1e-12
     n = lib.emu_plan_variant_count(op.h, kind)
     assert n >= 2
     for v in range(-1, n):
         lib.emu_plan_set_variant(op.h, kind, v)
-        assert _rel(op.matvec(u)[0], ref) < 1e-6, v
-        assert _rel(op.rmatvec(u)[0], refa) < 1e-6, v
+        assert _rel(op.matvec(u)[0], ref) < tol, v
+        assert _rel(op.rmatvec(u)[0], refa) < tol, v

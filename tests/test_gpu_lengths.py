@@ -181,36 +181,38 @@ def test_packed_members_vs_oracle(em):
             assert _rel(_host(got[j]), ref) < 1e-6, (rep, j)
 
 
-# (N, pass kind): x pass at 256 / 192 / 128, strided passes at 256 / 192,
-# z passes at 256 / 192 -- every variant the autotuner may pick, and the
-# generic kernels (-1)
-VARIANT_CASES = [((128, 2, 8), 1), ((96, 2, 8), 1), ((64, 2, 8), 1), ((2, 128, 8), 0), ((2, 96, 8), 0),
-                 ((2, 2, 128), 2), ((2, 2, 96), 2)]
+# (N, pass kind, precision): x pass at 256 / 192 / 128, strided passes at
+# 256 / 192, z passes at 256 / 192, the FP64 x pass at 256 -- every variant
+# the autotuner may pick, and the generic kernels (-1)
+VARIANT_CASES = [((128, 2, 8), 1, "single"), ((96, 2, 8), 1, "single"), ((64, 2, 8), 1, "single"),
+                 ((2, 128, 8), 0, "single"), ((2, 96, 8), 0, "single"), ((2, 2, 128), 2, "single"),
+                 ((2, 2, 96), 2, "single"), ((128, 2, 8), 1, "double")]
 
 
-@pytest.mark.parametrize("N,kind", VARIANT_CASES)
-def test_every_variant_vs_oracle_on_gpu(em, N, kind):
+@pytest.mark.parametrize("N,kind,precision", VARIANT_CASES)
+def test_every_variant_vs_oracle_on_gpu(em, N, kind, precision):
     from em3d_amd.operator import HipPlan
     from oracle import vie
     L, k = (1.0, 0.6, 0.8), 3.0
-    be = _be(em, "single")
+    be = _be(em, precision)
     chi = _rand((3, 3) + N, 21) * 0.3
     op = em.Operator(_em_problem(em, be, N, L, k, chi))
     u = _rand((3,) + N, 22)
-    ud = be.array(u.astype(np.complex64))
+    ud = be.array(u.astype(be.complex_dtype))
     Kh = vie.kernel_hat(N, L, k)
     ref, refa = vie.em_matvec(chi, u, Kh), vie.em_rmatvec(chi, u, Khat=Kh)
-    plan = HipPlan.get(N, 3, 1, em.Precision.SINGLE)
+    plan = HipPlan.get(N, 3, 1, be.precision)
+    tol = 1e-6 if precision == "single" else 1e-12
     n = plan.variant_counts()[kind]
     assert n >= 2
     try:
         for v in range(-1, n):
-            _say(f"[variant] N={N} kind={kind} variant={v}")
+            _say(f"[variant] N={N} kind={kind} precision={precision} variant={v}")
             vs = [0, 0, 0]
             vs[kind] = v
             plan.set_variants(vs)
-            assert _rel(_host(op.matvec(ud)), ref) < 1e-6, v
-            assert _rel(_host(op.rmatvec(ud)), refa) < 1e-6, v
+            assert _rel(_host(op.matvec(ud)), ref) < tol, v
+            assert _rel(_host(op.rmatvec(ud)), refa) < tol, v
     finally:
         plan.set_variants([0, 0, 0])
 
