@@ -78,91 +78,77 @@ __global__ __launch_bounds__(kBoundThreads) void k_bounds(int nx, int ny, int nz
 // The box's own axes: box axis a is grid axis perm[a] (the sides of the box
 // operator are n[perm[0]], n[perm[1]], n[perm[2]]); planes come in groups of
 // ncc = 1 (scalar), 3 (vector components, which follow the axes) or 9 (a
-// 3 x 3 tensor, both indices following the axes).
+// 3 x 3 tensor, both indices following the axes).  The strides and plane
+// maps are worked out on the host, so that the kernels index no array by a
+// run-time axis number.
 struct BoxMap {
-  int N[3];      // grid
-  int n[3];      // box sides along the grid axes
-  int lo[3];     // box offset along the grid axes
-  int perm[3];   // box axis a = grid axis perm[a]
+  int N[3];           // grid
+  int n[3];           // box sides along the grid axes
+  int lo[3];          // box offset along the grid axes
+  int perm[3];        // box axis a = grid axis perm[a]
   int ncc;
+  int bn[3];          // box sides in box-axis order, bn[k] = n[perm[k]]
+  long long mg, mb;   // cells of the grid and of the box
+  long long bst[3];   // box-layout stride of one step along grid axis a
+  long long gst[3];   // grid-layout stride of one step along box axis k
+  long long lo_off;   // grid index of the box's first cell
+  int gplane[9];      // grid plane of box plane q within its group
+  int bplane[9];      // box plane of grid plane g within its group
 };
 
-// grid plane of box plane q within its group
-__device__ __forceinline__ int grid_plane(const BoxMap& m, int q) {
-  if (m.ncc == 3) return m.perm[q];
-  if (m.ncc == 9) return 3 * m.perm[q / 3] + m.perm[q % 3];
-  return q;
+// Threads of a row kernel: W along the contiguous axis, R rows per block.
+struct RowShape {
+  int W, R;
+};
+inline RowShape row_shape(int len) {
+  RowShape r;
+  r.W = len < 256 ? len : 256;
+  r.R = 256 / r.W;
+  return r;
 }
 
-// dst[p][d0][d1][d2] (box axes) = src[grid plane][lo + d on the grid axes]
+// dst[p][d0][d1][d2] (box axes) = src[grid plane][lo + d on the grid axes]:
+// one thread per d2 of a box row, blockIdx.y the plane
 template <typename T>
-__global__ __launch_bounds__(256) void k_gather(long long nplanes, BoxMap m, const cpx<T>* src, cpx<T>* dst) {
-  const int b0 = m.n[m.perm[0]], b1 = m.n[m.perm[1]], b2 = m.n[m.perm[2]];
-  const long long mb = (long long)b0 * b1 * b2;
-  const long long n = nplanes * mb;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
-    const long long p = t / mb;
-    long long r = t - p * mb;
-    int d[3];
-    d[2] = (int)(r % b2);
-    r /= b2;
-    d[1] = (int)(r % b1);
-    d[0] = (int)(r / b1);
-    int g[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) g[m.perm[a]] = m.lo[m.perm[a]] + d[a];
+__global__ __launch_bounds__(256) void k_gather(BoxMap m, const cpx<T>* src, cpx<T>* dst) {
+  const long long p = blockIdx.y;
+  const int row = (int)blockIdx.x * (int)blockDim.y + (int)threadIdx.y;
+  if (row >= m.bn[0] * m.bn[1]) return;
+  const int d0 = row / m.bn[1], d1 = row - d0 * m.bn[1];
+  const long long grp = p / m.ncc;
+  const long long sp = grp * m.ncc + m.gplane[p - grp * m.ncc];
+  const cpx<T>* s = src + sp * m.mg + m.lo_off + d0 * m.gst[0] + d1 * m.gst[1];
+  cpx<T>* d = dst + p * m.mb + (long long)row * m.bn[2];
+  for (int d2 = threadIdx.x; d2 < m.bn[2]; d2 += blockDim.x) d[d2] = s[d2 * m.gst[2]];
+}
+
+// out[p][grid] = the box value inside the box; outside it f - y (f and y
+// given) or 0: one thread per z of a grid row, blockIdx.y the plane
+template <typename T>
+__global__ __launch_bounds__(256) void k_scatter(BoxMap m, const cpx<T>* box, const cpx<T>* f, const cpx<T>* y,
+                                                 cpx<T>* out) {
+  const long long p = blockIdx.y;
+  const int row = (int)blockIdx.x * (int)blockDim.y + (int)threadIdx.y;
+  if (row >= m.N[0] * m.N[1]) return;
+  const int g0 = row / m.N[1], g1 = row - g0 * m.N[1];
+  const int i0 = g0 - m.lo[0], i1 = g1 - m.lo[1];
+  const bool rin = i0 >= 0 && i0 < m.n[0] && i1 >= 0 && i1 < m.n[1];
+  const cpx<T>* bx = box;
+  if (rin) {
+    // the box plane whose grid plane is p's, within p's group
     const long long grp = p / m.ncc;
-    const long long sp = grp * m.ncc + grid_plane(m, (int)(p - grp * m.ncc));
-    dst[t] = src[((sp * m.N[0] + g[0]) * m.N[1] + g[1]) * (long long)m.N[2] + g[2]];
+    const long long bp = grp * m.ncc + m.bplane[p - grp * m.ncc];
+    bx = box + bp * m.mb + i0 * m.bst[0] + i1 * m.bst[1];
   }
-}
-
-// out[p][grid] = the box value inside the box; outside it f - y (f and y given) or 0
-template <typename T>
-__global__ __launch_bounds__(256) void k_scatter(long long nplanes, BoxMap m, const cpx<T>* box, const cpx<T>* f,
-                                                 const cpx<T>* y, cpx<T>* out) {
-  const long long mg = (long long)m.N[0] * m.N[1] * m.N[2];
-  const int b1 = m.n[m.perm[1]], b2 = m.n[m.perm[2]];
-  const long long n = nplanes * mg;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
-    const long long p = t / mg;
-    long long r = t - p * mg;
-    int g[3];
-    g[2] = (int)(r % m.N[2]);
-    r /= m.N[2];
-    g[1] = (int)(r % m.N[1]);
-    g[0] = (int)(r / m.N[1]);
-    int i[3];
-    bool in = true;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      i[a] = g[a] - m.lo[a];
-      in = in && i[a] >= 0 && i[a] < m.n[a];
-    }
+  const long long base = p * m.mg + (long long)row * m.N[2];
+  for (int g2 = threadIdx.x; g2 < m.N[2]; g2 += blockDim.x) {
+    const int i2 = g2 - m.lo[2];
     cpx<T> v;
-    if (in) {
-      // the box plane whose grid plane is p's, within p's group
-      const long long grp = p / m.ncc;
-      const int gp = (int)(p - grp * m.ncc);
-      int q = gp;
-      for (int c = 0; c < m.ncc; ++c)
-        if (grid_plane(m, c) == gp) q = c;
-      const long long bp = grp * m.ncc + q;
-      v = box[((bp * m.n[m.perm[0]] + i[m.perm[0]]) * b1 + i[m.perm[1]]) * (long long)b2 + i[m.perm[2]]];
-    } else if (f) {
-      v = f[t] - y[t];
-    } else {
-      v = czero<T>();
-    }
-    out[t] = v;
+    if (rin && i2 >= 0 && i2 < m.n[2]) v = bx[i2 * m.bst[2]];
+    else if (f) v = f[base + g2] - y[base + g2];
+    else v = czero<T>();
+    out[base + g2] = v;
   }
-}
-
-unsigned grid_for(long long n) {
-  const long long want = (n + 255) / 256;
-  return (unsigned)(want < 1 ? 1 : (want > 65536 ? 65536 : want));
 }
 
 int make_map(const char* who, long long nplanes, int ncc, const int* grid, const int* box, const int* lo,
@@ -183,7 +169,35 @@ int make_map(const char* who, long long nplanes, int ncc, const int* grid, const
     m->perm[a] = perm[a];
   }
   m->ncc = ncc;
+  if (nplanes > 65535) return sfail(std::string(who) + ": at most 65535 planes per call");
+  if ((long long)grid[0] * grid[1] > INT_MAX) return sfail(std::string(who) + ": grid too large");
+  long long gstride[3];   // grid-layout stride of grid axis a
+  gstride[2] = 1;
+  gstride[1] = grid[2];
+  gstride[0] = (long long)grid[1] * grid[2];
+  for (int k = 0; k < 3; ++k) m->bn[k] = box[perm[k]];
+  m->mg = (long long)grid[0] * grid[1] * grid[2];
+  m->mb = (long long)box[0] * box[1] * box[2];
+  const long long bstride_k[3] = {(long long)m->bn[1] * m->bn[2], m->bn[2], 1};   // box axis k
+  for (int k = 0; k < 3; ++k) {
+    m->bst[perm[k]] = bstride_k[k];
+    m->gst[k] = gstride[perm[k]];
+  }
+  m->lo_off = lo[0] * gstride[0] + lo[1] * gstride[1] + lo[2];
+  for (int q = 0; q < 9; ++q) m->gplane[q] = m->bplane[q] = 0;
+  for (int q = 0; q < ncc; ++q) {
+    const int g = ncc == 3 ? perm[q] : ncc == 9 ? 3 * perm[q / 3] + perm[q % 3] : q;
+    m->gplane[q] = g;
+    m->bplane[g] = q;
+  }
   return 0;
+}
+
+// launch shape of a row kernel over `rows` rows of length `len`, `planes` planes
+void row_launch(int rows, int len, long long planes, dim3* grid, dim3* block) {
+  const RowShape r = row_shape(len);
+  *block = dim3(r.W, r.R);
+  *grid = dim3((unsigned)((rows + r.R - 1) / r.R), (unsigned)planes);
 }
 
 }  // namespace
@@ -221,14 +235,13 @@ int em3d_support_gather(int precision, long long nplanes, int ncc, const int* gr
   if (!src || !dst) return sfail("em3d_support_gather: null argument");
   BoxMap m;
   if (int rc = make_map("em3d_support_gather", nplanes, ncc, grid, box, lo, perm, &m)) return rc;
-  const long long n = nplanes * m.n[0] * m.n[1] * m.n[2];
-  if (n == 0) return 0;
+  if (nplanes == 0) return 0;
+  dim3 g, blk;
+  row_launch(m.bn[0] * m.bn[1], m.bn[2], nplanes, &g, &blk);
   if (precision == EM3D_DOUBLE)
-    hipLaunchKernelGGL(k_gather<double>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, m,
-                       (const cpx<double>*)src, (cpx<double>*)dst);
+    hipLaunchKernelGGL(k_gather<double>, g, blk, 0, stream, m, (const cpx<double>*)src, (cpx<double>*)dst);
   else
-    hipLaunchKernelGGL(k_gather<float>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, m, (const cpx<float>*)src,
-                       (cpx<float>*)dst);
+    hipLaunchKernelGGL(k_gather<float>, g, blk, 0, stream, m, (const cpx<float>*)src, (cpx<float>*)dst);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sfail(std::string("em3d_support_gather: ") + hipGetErrorString(e));
   return 0;
@@ -241,14 +254,15 @@ int em3d_support_scatter(int precision, long long nplanes, int ncc, const int* g
   if ((f == nullptr) != (y == nullptr)) return sfail("em3d_support_scatter: give both f and y, or neither");
   BoxMap m;
   if (int rc = make_map("em3d_support_scatter", nplanes, ncc, grid, box, lo, perm, &m)) return rc;
-  const long long n = nplanes * m.N[0] * m.N[1] * m.N[2];
-  if (n == 0) return 0;
+  if (nplanes == 0) return 0;
+  dim3 g, blk;
+  row_launch(m.N[0] * m.N[1], m.N[2], nplanes, &g, &blk);
   if (precision == EM3D_DOUBLE)
-    hipLaunchKernelGGL(k_scatter<double>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, m,
-                       (const cpx<double>*)boxed, (const cpx<double>*)f, (const cpx<double>*)y, (cpx<double>*)out);
+    hipLaunchKernelGGL(k_scatter<double>, g, blk, 0, stream, m, (const cpx<double>*)boxed, (const cpx<double>*)f,
+                       (const cpx<double>*)y, (cpx<double>*)out);
   else
-    hipLaunchKernelGGL(k_scatter<float>, dim3(grid_for(n)), dim3(256), 0, stream, nplanes, m,
-                       (const cpx<float>*)boxed, (const cpx<float>*)f, (const cpx<float>*)y, (cpx<float>*)out);
+    hipLaunchKernelGGL(k_scatter<float>, g, blk, 0, stream, m, (const cpx<float>*)boxed, (const cpx<float>*)f,
+                       (const cpx<float>*)y, (cpx<float>*)out);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sfail(std::string("em3d_support_scatter: ") + hipGetErrorString(e));
   return 0;
