@@ -48,6 +48,7 @@ SIGNATURES = {
     "em3d_matvec": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
     "em3d_rmatvec": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, _VP]),
     "em3d_matvec_noalloc": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP]),
+    "em3d_matvec_window": (_I, [_VP, _VP, _VP, _I, _VP, _VP, _VP, C.POINTER(_I), C.POINTER(_I), _VP]),
     "em3d_plan_set_timing": (_I, [_VP, _I]),
     "em3d_plan_timing": (_I, [_VP, C.POINTER(_D), _I, C.POINTER(_I), C.POINTER(_LL)]),
     "em3d_group_create": (_I, [_I, _VP, _VP, _VP, _VP, _VP, _VP, _VP, C.POINTER(_VP)]),

@@ -210,7 +210,20 @@ struct ZArgs {
   int nt;              // fast kernels: rows read and written non-temporally
   const int* skip;             // device flag: when non-zero the whole pass is a no-op
   ZKry k;                      // fused Krylov step (k.op == KRY_NONE: none)
+  // input window of a forward pass (wn1 == 0: none): logical pencil q of the
+  // P = n0 n1 pencils is grid pencil wbase + (q / wn1) wny + q % wn1, and
+  // input z outside [wz0, wz1) reads as zero
+  int wn1, wny, wz0, wz1;
+  long long wbase;
 };
+
+// grid pencil of logical pencil q, and whether input z lies in the window
+E3_HD long long z_pencil(const ZArgs& a, long long q) {
+  if (!a.wn1) return q;
+  const int qi = (int)q, r = qi / a.wn1;
+  return a.wbase + (long long)r * a.wny + (qi - r * a.wn1);
+}
+E3_HD bool z_live(const ZArgs& a, int z) { return !a.wz1 || (z >= a.wz0 && z < a.wz1); }
 
 // The Krylov prologue value of component c at element o (offset within the
 // right-hand side b), before the contrast: KRY_P p, KRY_S s (and |s|^2 into
@@ -271,9 +284,9 @@ struct ZPass {
     const bool kry = (a.k.op == KRY_P || a.k.op == KRY_S) && !a.k.coef[b].done;
     for (int e = tid; e < a.T * L; e += nt) {
       const int t = e / L, z = e - t * L;
-      const long long p = p0 + t;
+      const long long p = z_pencil(a, p0 + t);
       cpx<T> w[6];
-      if (p < a.P && z < a.n_in) {
+      if (p0 + t < a.P && z < a.n_in && z_live(a, z)) {
         cpx<T> v[6];
         const int zi = (a.perm && a.inv) ? mirror_position(z, a.n_in / 2, a.n_in) : z;
         for (int c = 0; c < a.C; ++c) {
@@ -299,8 +312,8 @@ struct ZPass {
     const cpx<T>* src = (const cpx<T>*)a.u + (size_t)b * a.out_bs;
     for (int e = tid; e < a.T * a.n_out; e += nt) {
       const int t = e / a.n_out, z = e - t * a.n_out;
-      const long long p = p0 + t;
-      if (p >= a.P) continue;
+      if (p0 + t >= a.P) continue;
+      const long long p = z_pencil(a, p0 + t);
       cpx<T> v[6];
       for (int c = 0; c < a.C; ++c) v[c] = res[(size_t)(c * a.T + t) * a.ps + z];
       if (a.epi == EPI_SUB_CHIH) {
@@ -340,7 +353,18 @@ struct SArgs {
   int variant;         // fast-kernel variant chosen by the plan (-1: the generic kernels)
   int nt;              // fast kernels: rows read and written non-temporally
   const int* skip;             // device flag: when non-zero the whole pass is a no-op
+  // input window of a forward pass (wn == 0: none): logical plane q of Q is
+  // plane (q / wn) wnq + wlo + q % wn, and input rows outside [wr0, wr1)
+  // read as zero
+  int wn, wnq, wlo, wr0, wr1;
 };
+
+E3_HD long long s_plane(const SArgs& a, long long q) {
+  if (!a.wn) return q;
+  const long long r = q / a.wn;
+  return r * a.wnq + a.wlo + (q - r * a.wn);
+}
+E3_HD bool s_live(const SArgs& a, int row) { return !a.wr1 || (row >= a.wr0 && row < a.wr1); }
 
 template <typename T>
 struct SPass {
@@ -349,7 +373,7 @@ struct SPass {
   cpx<T>* b0;
   cpx<T>* b1;
   E3_HD SPass(const SArgs& args, long long colblk, long long plane, cpx<T>* lds)
-      : a(args), q(plane), w0(colblk * args.T), b0(lds), b1(lds + (size_t)args.T * args.ps) {}
+      : a(args), q(s_plane(args, plane)), w0(colblk * args.T), b0(lds), b1(lds + (size_t)args.T * args.ps) {}
 
   E3_HD void load(int tid, int nt) const {
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)q * a.in_qs;
@@ -358,7 +382,7 @@ struct SPass {
       const int row = e / a.T, col = e - row * a.T;
       const long long w = w0 + col;
       cpx<T> v = czero<T>();
-      if (row < a.n_in && w < a.W) v = in[(size_t)row * a.W + w];
+      if (row < a.n_in && w < a.W && s_live(a, row)) v = in[(size_t)row * a.W + w];
       b0[(size_t)col * a.ps + row] = v;
     }
   }
@@ -396,7 +420,10 @@ struct XArgs {
   int variant;         // fast-kernel variant chosen by the plan (-1: the generic kernels)
   int nt;              // fast kernels: rows read and written non-temporally
   const int* skip;     // device flag: when non-zero the whole pass is a no-op
+  int wx0, wx1;        // input window of the forward transform (wx1 == 0: none): rows outside read as zero
 };
+
+E3_HD bool x_live(const XArgs& a, int x) { return !a.wx1 || (x >= a.wx0 && x < a.wx1); }
 
 // ---------------------------------------------------------------------------
 // Octant spectrum layout.  Entries (comp, ix, iy, iz), 0 <= i* <= N*, are
@@ -473,7 +500,7 @@ struct XPass {
       const int row = r / a.T, col = r - row * a.T;
       const long long w = w0 + col;
       cpx<T> v = czero<T>();
-      if (row < a.N[0] && w < a.W) v = d[(size_t)c * a.cs + (size_t)row * a.W + w];
+      if (row < a.N[0] && w < a.W && x_live(a, row)) v = d[(size_t)c * a.cs + (size_t)row * a.W + w];
       b0[(size_t)(c * a.T + col) * a.ps + row] = v;
     }
   }

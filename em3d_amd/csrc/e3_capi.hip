@@ -191,7 +191,7 @@ void set_lds_limits_once() {
 
 int e3::apply_plan(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
                    void* work, hipStream_t stream, bool adjoint, const char* name, const int* skip,
-                   const ZKry* k1, const ZKry* k5, const int* skip1, unsigned passes) {
+                   const ZKry* k1, const ZKry* k5, const int* skip1, unsigned passes, const int* win) {
   if (!plan || !khat || !u || !y || !work) return fail(std::string(name) + ": null argument");
   if (chi_kind < CHI_NONE || chi_kind > CHI_FULL) return fail(std::string(name) + ": bad chi_kind");
   if (chi_kind != CHI_NONE && !chi) return fail(std::string(name) + ": null chi");
@@ -201,7 +201,13 @@ int e3::apply_plan(em3d_plan_t plan, const void* khat, const void* chi, int chi_
   if (plan->timing) go.ev = plan->ev;
   if ((k1 && k1->op != KRY_NONE && (chi_kind == CHI_FULL || adjoint)) || (k5 && k5->op != KRY_NONE && adjoint))
     return fail(std::string(name) + ": fused Krylov steps need a matvec with a non-mixing contrast");
-  run_apply(go, plan->p, khat, chi, chi_kind, u, y, work, adjoint, skip, k1, k5, skip1, passes);
+  if (win) {
+    if (k1 && k1->op != KRY_NONE) return fail(std::string(name) + ": an input window takes no fused Krylov step");
+    for (int a = 0; a < 3; ++a)
+      if (win[a] < 0 || win[3 + a] < 1 || win[a] + win[3 + a] > plan->p.N[a])
+        return fail(std::string(name) + ": the window must be a non-empty box inside the grid");
+  }
+  run_apply(go, plan->p, khat, chi, chi_kind, u, y, work, adjoint, skip, k1, k5, skip1, passes, win);
   if (plan->timing && go.launches <= kMaxTimedLaunches) {
     go.note(hipEventRecord(plan->ev[go.launches], stream));
     go.note(hipEventSynchronize(plan->ev[go.launches]));
@@ -527,6 +533,14 @@ int em3d_matvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kin
 int em3d_rmatvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* v, void* y,
                  void* work, hipStream_t stream) {
   return e3::apply_plan(plan, khat, chi, chi_kind, v, y, work, stream, true, "em3d_rmatvec", nullptr);
+}
+
+int em3d_matvec_window(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
+                       void* work, const int* lo, const int* n, hipStream_t stream) {
+  if (!lo || !n) return fail("em3d_matvec_window: null window");
+  const int win[6] = {lo[0], lo[1], lo[2], n[0], n[1], n[2]};
+  return e3::apply_plan(plan, khat, chi, chi_kind, u, y, work, stream, false, "em3d_matvec_window", nullptr,
+                        nullptr, nullptr, nullptr, e3::PASSES_ALL, win);
 }
 
 }  // extern "C"

@@ -342,7 +342,7 @@ struct FastS {
   long long q, w0;
   cpx<T>* lds;
   E3_HD FastS(const SArgs& args, long long colblk, long long plane, cpx<T>* l)
-      : a(args), q(plane), w0(colblk * TC), lds(l) {}
+      : a(args), q(s_plane(args, plane)), w0(colblk * TC), lds(l) {}
 
   E3_HD void stage_a(int tid, St& s) const {
     const int col = tid % TC, j = tid / TC;
@@ -352,7 +352,9 @@ struct FastS {
     const unsigned lane = (unsigned)(((long long)j * a.W + w) * sizeof(cpx<T>));
 #pragma unroll
     for (int m = 0; m < R1; ++m)
-      s.v[m] = (m < MA && ok) ? ld_nt<NTS && (INV ? E3_SI_LD_NT : E3_SF_LD_NT)>(at_offset(in + (size_t)m * R2 * a.W, lane)) : czero<T>();
+      s.v[m] = (m < MA && ok && s_live(a, j + m * R2))
+                   ? ld_nt<NTS && (INV ? E3_SI_LD_NT : E3_SF_LD_NT)>(at_offset(in + (size_t)m * R2 * a.W, lane))
+                   : czero<T>();
     fft_reg<T, R1, INV>(s.v);
 #pragma unroll
     for (int k = 0; k < R1; ++k) lds[(k * R2 + j) * TCP + col] = s.v[k];
@@ -576,7 +578,9 @@ struct FastX {
       cpx<T> v[RA];
 #pragma unroll
       for (int m = 0; m < RA; ++m)
-        v[m] = (m < RA / 2 && lv) ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) * sg : czero<T>();
+        v[m] = (m < RA / 2 && lv && x_live(a, jj + m * (L / RA)))
+                   ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) * sg
+                   : czero<T>();
       fft_reg<T, RA, false>(v);
 #pragma unroll
       for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
@@ -595,7 +599,9 @@ struct FastX {
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
 #pragma unroll
       for (int m = 0; m < RA / 2; ++m)
-        s.pre[bb][m] = lv ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) : czero<T>();
+        s.pre[bb][m] = (lv && x_live(a, jj + m * (L / RA)))
+                           ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b))
+                           : czero<T>();
     }
   }
   E3_HD void fwd_a_pre(int c, int tid, int h, St& s, int buf) const {
@@ -1008,11 +1014,12 @@ struct FastZF {
     int j = tid % R2, pl = tid / R2;
     E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
     E3_OPAQUE(pl);
-    const long long p = p0 + pl;
-    const bool ok = p < a.P;
+    const bool ok = p0 + pl < a.P;
+    const long long p = z_pencil(a, p0 + pl);
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs;
 #pragma unroll
-    for (int m = 0; m < R1; ++m) v[m] = (m < MA && ok) ? contrast_row<T, NTZ, NTU>(a, in, c, p, j + m * R2) : czero<T>();
+    for (int m = 0; m < R1; ++m)
+      v[m] = (m < MA && ok && z_live(a, j + m * R2)) ? contrast_row<T, NTZ, NTU>(a, in, c, p, j + m * R2) : czero<T>();
   }
   // stage A input with a fused Krylov prologue (KRY_P / KRY_S, iso or diag
   // contrast): the update of each live element, written back, then chi.  The
@@ -1022,11 +1029,11 @@ struct FastZF {
     int j = tid % R2, pl = tid / R2;
     E3_OPAQUE(j);
     E3_OPAQUE(pl);
-    const long long p = p0 + pl;
     const KryCoef k = a.k.coef[b];
 #pragma unroll
     for (int m = 0; m < R1; ++m) v[m] = czero<T>();
-    if (p >= a.P || k.done) return;
+    if (p0 + pl >= a.P || k.done) return;
+    const long long p = z_pencil(a, p0 + pl);
     const size_t row = (size_t)b * a.in_bs + (size_t)c * a.in_cs + (size_t)p * a.in_ps;
     const cpx<T>* chi = nullptr;
     if (a.pro == PRO_CHI && a.chi_kind != CHI_NONE)
@@ -1057,7 +1064,8 @@ struct FastZF {
     int j = tid % R2, pl = tid / R2;
     E3_OPAQUE(j);   // lane addresses are formed per stage, not held across components
     E3_OPAQUE(pl);
-    const long long p = p0 + pl;
+    const bool ok = p0 + pl < a.P;
+    const long long p = z_pencil(a, p0 + pl);
     const cpx<T>* tw = (const cpx<T>*)a.tw;
     cpx<T>* out = (cpx<T>*)a.out + (size_t)b * a.out_bs + (size_t)c * a.out_cs + (size_t)p * a.out_ps;
 #pragma unroll
@@ -1070,7 +1078,7 @@ struct FastZF {
         v[r] = r ? twiddle<T, false>(tw, jp * r, x) : x;
       }
       fft_reg<T, R2, false>(v);
-      if (p < a.P) {
+      if (ok) {
 #pragma unroll
         for (int m = 0; m < R2; ++m)
           st_nt<E3_Z_NT_ST>(&out[a.perm ? mirror_position(jp + m * R1, L / 2, L) : jp + m * R1], v[m]);

@@ -110,11 +110,15 @@ enum : unsigned { PASSES_ALL = 0x1Fu, PASSES_FIRST = 0x01u, PASSES_REST = 0x1Eu 
 // y = A u (adjoint = false) or y = A^H u (adjoint = true) for all batch RHS.
 // k1 / k5: fused Krylov steps of the first / last z pass (device BiCGStab);
 // skip: device flag of every pass run here (skip1, when given, of P1).
+// win (lo[3], n[3]; null: none): the forward passes read only the cells of
+// that box, i.e. the transformed field (chi u, or u for the adjoint) is taken
+// as zero outside it.  P1 runs on the box's n0 n1 pencils, P2 on its n0 x
+// planes and n1 rows, P3 on its n0 rows; P4 and P5 cover the whole grid.
 template <class Launcher>
 void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, int chi_kind,
                const void* u, void* y, void* work, bool adjoint, const int* skip = nullptr,
                const ZKry* k1 = nullptr, const ZKry* k5 = nullptr, const int* skip1 = nullptr,
-               unsigned passes = PASSES_ALL) {
+               unsigned passes = PASSES_ALL, const int* win = nullptr) {
   const int prec = p.prec;
   const size_t es = elem_bytes(prec);
   void* W1 = work;
@@ -140,6 +144,11 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   z.variant = p.variant[PASS_Z];
   z.nt = nt;
   if (k1) z.k = *k1;
+  if (win) {
+    z.P = (long long)win[3] * win[4];
+    z.wn1 = win[4]; z.wny = Ny; z.wbase = (long long)win[0] * Ny + win[1];
+    z.wz0 = win[2]; z.wz1 = win[2] + win[5];
+  }
   if (passes & 0x01u) z_pass(go, p, prec, twz, p.ax[2], z);
 
   // P2: y forward (Ny live rows -> Ly)
@@ -148,7 +157,14 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   s.in = W1; s.out = W2; s.Q = (long long)p.batch * C * Nx; s.W = Lz;
   s.in_qs = (long long)Ny * Lz; s.out_qs = (long long)Ly * Lz;
   s.n_in = Ny; s.n_out = Ly; s.inv = 0; s.skip = skip; s.variant = p.variant[PASS_S]; s.nt = nt;
+  if (win) {
+    s.Q = (long long)p.batch * C * win[3];
+    s.wn = win[3]; s.wnq = Nx; s.wlo = win[0];
+    s.wr0 = win[1]; s.wr1 = win[1] + win[4];
+  }
   if (passes & 0x02u) s_pass(go, prec, twy, p.ax[1], s);
+  s.Q = (long long)p.batch * C * Nx;
+  s.wn = s.wnq = s.wlo = s.wr0 = s.wr1 = 0;
 
   // P3: x forward, spectral multiply, x inverse (Nx rows kept), in place
   XArgs x;
@@ -158,6 +174,9 @@ void run_apply(Launcher& go, const Plan& p, const void* khat, const void* chi, i
   x.plan = p.ax[0]; x.tw = twx; x.khat = khat; x.adj = adjoint ? 1 : 0; x.batch = p.batch; x.perm = 1; x.skip = skip;
   x.variant = p.variant[PASS_X];
   x.nt = nt;
+  if (win) {
+    x.wx0 = win[0]; x.wx1 = win[0] + win[3];
+  }
   x.ps = Lx + 1;
   x.T = tile_count(lds_bytes(p.C, x.ps, prec), 16);
   if (x.T > x.W) x.T = (int)x.W;

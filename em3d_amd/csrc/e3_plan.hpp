@@ -30,10 +30,11 @@ namespace e3 {
 // y = A u (or A^H u) on the plan's stream; when skip is non-null every kernel
 // returns at once while *skip is non-zero (device-side early exit).  k1 / k5:
 // fused Krylov steps of the first / last z pass; skip1: P1's flag when given;
-// passes: which of P1..P5 to run (PASSES_ALL, PASSES_FIRST, PASSES_REST).
+// passes: which of P1..P5 to run (PASSES_ALL, PASSES_FIRST, PASSES_REST);
+// win: the input box of the forward passes (run_apply), validated here.
 int apply_plan(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
                void* work, hipStream_t stream, bool adjoint, const char* name, const int* skip,
                const ZKry* k1 = nullptr, const ZKry* k5 = nullptr, const int* skip1 = nullptr,
-               unsigned passes = PASSES_ALL);
+               unsigned passes = PASSES_ALL, const int* win = nullptr);
 
 }  // namespace e3

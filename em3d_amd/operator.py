@@ -448,17 +448,28 @@ class _FieldApplier:
                    "em3d_plan_autotune")
         return {"variants": plan.variants(), "ms": best.value}
 
-    def _apply(self, field, khat, chi_kind, chi, adjoint: bool):
+    def _apply(self, field, khat, chi_kind, chi, adjoint: bool, window=None):
         import torch
         t, batch = self._prepare_input(field)
         plan = HipPlan.get(self._N, self._ncomp, batch, self._be.precision, self._be.device_index)
         y = torch.empty_like(t)
+        args = (plan.handle, khat.data_ptr(), chi.data_ptr() if chi is not None else None, chi_kind,
+                t.data_ptr(), y.data_ptr(), self._workspace(plan).data_ptr())
+        stream = _lib.stream_handle(self._be.torch_device)
+        if window is not None:
+            lo, n = (C.c_int * 3)(*[int(v) for v in window[0]]), (C.c_int * 3)(*[int(v) for v in window[1]])
+            _lib.check(_lib.lib().em3d_matvec_window(*args, lo, n, stream), "em3d_matvec_window")
+            return y
         fn = _lib.lib().em3d_rmatvec if adjoint else _lib.lib().em3d_matvec
-        _lib.check(fn(plan.handle, khat.data_ptr(), chi.data_ptr() if chi is not None else None, chi_kind,
-                      t.data_ptr(), y.data_ptr(), self._workspace(plan).data_ptr(),
-                      _lib.stream_handle(self._be.torch_device)),
-                   "em3d_rmatvec" if adjoint else "em3d_matvec")
+        _lib.check(fn(*args, stream), "em3d_rmatvec" if adjoint else "em3d_matvec")
         return y
+
+    def matvec_window(self, field, lo, n):
+        """``matvec`` with the forward transforms restricted to the box of cells
+        [lo, lo + n) (em3d_matvec_window; no counterpart in the reference): chi u
+        is taken as zero outside the box, so the result equals ``matvec``
+        whenever the box holds the contrast's support (or u's)."""
+        return self._apply(field, *self._spectrum(), False, window=(lo, n))
 
 
 class _SingleCompanion(_FieldApplier):

@@ -6,6 +6,8 @@ No counterpart in the reference, whose solvers iterate on the whole grid
 (``em3d_amd.support``: the same operator on the box of cells that holds
 supp(chi)), then forms the whole field with one matvec of the full operator:
 u = u_box inside the box, u = f - A u~ outside (u~ = u_box extended by zero).
+That matvec transforms forward only the box (``matvec_window``): chi u~ is
+zero everywhere else.
 The full grid's residual equals the box's inside the box and vanishes
 outside, so the inner solve's tolerance is rescaled to the full right-hand
 side: ``|r_box| < max(rtol |f|, atol)``.  Histories are reported relative to
@@ -58,7 +60,8 @@ class SupportRestricted:
                                 extras={"support": info})
         scale = fbn / fn
         res = self._inner(float(self.solver.cfg.rtol) / scale).solve(sub, fb)
-        y = operator.matvec(sub.extend(res.u))              # A u~, u~ = u_box extended by zero
+        # A u~ (u~ = u_box extended by zero), its forward transforms on the box alone
+        y = operator.matvec_window(sub.extend(res.u), sub.box.lo, sub.box.n)
         u = sub.extend(res.u, f, y, out=y)                   # u_box inside, f - A u~ outside
         extras = dict(res.extras or {})
         extras["support"] = info
@@ -89,7 +92,7 @@ class SupportRestricted:
         # one batched solve takes one rtol: the tightest of the rescaled ones
         res = self._inner(min(float(self.solver.cfg.rtol) / s for s in scale)).solve_batch(sub, fb)
         ub = torch.stack([r.u for r in res])
-        y = operator.matvec(sub.extend(ub))
+        y = operator.matvec_window(sub.extend(ub), sub.box.lo, sub.box.n)
         u = sub.extend(ub, f, y, out=y)
         info = {"box": list(sub.box.n), "offset": list(sub.box.lo), "order": list(sub.box.perm),
                 "fraction": round(sub.box.fraction, 5)}

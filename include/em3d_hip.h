@@ -78,6 +78,17 @@ int em3d_matvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kin
 int em3d_rmatvec(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* v, void* y,
                  void* work, hipStream_t stream);
 
+/* em3d_matvec whose forward transforms read only the box of cells
+ * [lo, lo + n) (lo[a] >= 0, n[a] >= 1, lo[a] + n[a] <= N[a]): chi u is taken
+ * as zero outside it, and y = u - B(chi u) is formed on the whole grid.  It
+ * equals em3d_matvec whenever chi u vanishes outside the box, e.g. when the box
+ * holds supp(chi), and then skips the forward work on the rest of the grid.
+ * No counterpart in the reference, whose operator always transforms the whole
+ * grid (operator.py:353-363); its caller here is the support-restricted
+ * solve, whose final field is f - A u~ with u~ zero outside the contrast's box. */
+int em3d_matvec_window(em3d_plan_t plan, const void* khat, const void* chi, int chi_kind, const void* u, void* y,
+                       void* work, const int* lo, const int* n, hipStream_t stream);
+
 /* em3d_matvec without a caller workspace: the FFT workspace is a fixed
  * file-scope device array of EM3D_NOALLOC_SCRATCH_BYTES per GPU (held until
  * the process ends), so the call allocates nothing.  No counterpart in the

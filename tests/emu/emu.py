@@ -41,6 +41,8 @@ def load(path: str | None = None):
     lib.emu_apply_kry.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p]
+    lib.emu_apply_window.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]
     lib.emu_last_error.restype = C.c_char_p
     lib.emu_plan_variant_count.argtypes = [C.c_void_p, C.c_int]
     lib.emu_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -83,6 +85,15 @@ class EmuOperator:
 
     def matvec(self, u):
         return self._apply(u, False)
+
+    def matvec_window(self, u, lo, n):
+        """A u with the forward passes restricted to the box [lo, lo + n)."""
+        u = np.ascontiguousarray(u, dtype=self.cdt)
+        y = np.empty_like(u)
+        win = np.array(list(lo) + list(n), dtype=np.int32)
+        self.lib.emu_apply_window(self.h, self.khat.ctypes.data, self.chi.ctypes.data, self.chi_kind,
+                                  u.ctypes.data, y.ctypes.data, self.work.ctypes.data, win.ctypes.data)
+        return y
 
     def rmatvec(self, v):
         return self._apply(v, True)

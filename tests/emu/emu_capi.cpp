@@ -326,6 +326,15 @@ void emu_apply(void* h, const void* khat, const void* chi, int chi_kind, const v
   run_apply(go, ((EmuPlan*)h)->p, khat, chi, chi_kind, u, y, work, adjoint != 0);
 }
 
+// y = A u with the forward passes restricted to the box [lo, lo + n) (win =
+// lo[3], n[3]), as em3d_matvec_window
+void emu_apply_window(void* h, const void* khat, const void* chi, int chi_kind, const void* u, void* y, void* work,
+                      const int* win) {
+  EmuLauncher go;
+  run_apply(go, ((EmuPlan*)h)->p, khat, chi, chi_kind, u, y, work, false, nullptr, nullptr, nullptr, nullptr,
+            PASSES_ALL, win);
+}
+
 // y = A u with the fused Krylov steps of the device BiCGStab: op1 (KRY_P /
 // KRY_S, updating u in place from r and v) in the first z pass, op5 (KRY_DOT
 // / KRY_DOT2) in the last; partial sums of P1 / P5 to part1 / part5.

@@ -231,6 +231,72 @@ def _oracle_op(chi, Kh, acoustic=False):
     return Op()
 
 
+def _boxed(shape, lo, n, seed, lead=()):
+    """Random values inside the box [lo, lo + n) of the last three axes, zero outside."""
+    x = np.zeros(lead + shape, complex)
+    sl = tuple(slice(lo[a], lo[a] + n[a]) for a in range(3))
+    x[(Ellipsis,) + sl] = _rand(lead + tuple(n), seed)
+    return x
+
+
+# (N, lo, n): boxes inside, filling, on a face and along an edge of grids whose
+# doubled axes take the register kernels (64, 48, 40 / 96, 80, 48 points)
+WINDOW_CASES = [((32, 24, 20), (5, 3, 2), (20, 15, 12)), ((32, 24, 20), (0, 0, 0), (32, 24, 20)),
+                ((32, 24, 20), (31, 0, 7), (1, 24, 1)), ((48, 40, 24), (10, 5, 3), (30, 30, 15))]
+
+
+@pytest.mark.parametrize("prec,tol", [("double", 1e-12), ("single", 1e-6)])
+@pytest.mark.parametrize("kind", ["diag", "full"])
+@pytest.mark.parametrize("N,lo,n", WINDOW_CASES)
+def test_window_matvec(em, prec, tol, kind, N, lo, n):
+    """em3d_matvec_window with chi zero outside the box (u is not): the same
+    bits as em3d_matvec, whose forward passes transform exact zeros there, and
+    the oracle's matvec (operator.py:353-363)."""
+    from oracle import vie
+    _say(f"[window matvec] N={N} lo={lo} n={n} kind={kind} prec={prec}")
+    be = _be(em, prec)
+    L, k = (1.0, 0.8, 1.2), 3.0
+    chi = _boxed(N, lo, n, 1, (3, 3)) * 0.3
+    if kind == "diag":
+        chi *= np.eye(3)[:, :, None, None, None]
+    grid = em.Grid(N=N, L=L, center=(0.0, 0.0, 0.0), backend=be)
+    prob = em.Problem(grid=grid, eps_tensor=be.array(chi.astype(be.complex_dtype)),
+                      wave=be.array(_rand((3,) + N, 2).astype(be.complex_dtype)), k0=k, volume=1.0)
+    op = em.Operator(prob)
+    u = _rand((3,) + N, 3)
+    ud = be.array(u.astype(be.complex_dtype))
+    yw = _host(op.matvec_window(ud, lo, n))
+    assert np.array_equal(yw, _host(op.matvec(ud)))
+    assert _rel(yw, vie.em_matvec(chi, u, vie.kernel_hat(N, L, k))) < tol
+
+
+def test_window_matvec_full_size_and_acoustic(em):
+    """At 128^3 (the bench's ellipsoid, its support box) and for the scalar
+    operator: the windowed matvec gives the same bits as the whole-grid one;
+    a box outside the grid is refused."""
+    import em3d_amd._lib as lib
+    from em3d_amd.support import support_operator
+    _say("[window matvec] 128^3 ellipsoid box, FP64 and complex64; acoustic 40^3")
+    for prec in ("double", "single"):
+        be = _be(em, prec)
+        prob = _em_problem(em, be, (128, 128, 128), 8.0, "diag", center=(0.0, 0.0, 0.0), radii=(0.3, 0.2, 0.15))
+        op = em.Operator(prob)
+        box = support_operator(op).box
+        u = be.array(_rand((3, 128, 128, 128), 4).astype(be.complex_dtype))
+        assert np.array_equal(_host(op.matvec_window(u, box.lo, box.n)), _host(op.matvec(u)))
+        with pytest.raises(lib.HipLibraryError):
+            op.matvec_window(u, (100, 0, 0), (40, 8, 8))
+    N, lo, n = (40, 40, 40), (6, 11, 2), (25, 20, 30)
+    be = _be(em, "double")
+    grid = em.Grid(N=N, L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+    eta = 1.0 + _boxed(N, lo, n, 5) * 0.4
+    prob = em.acoustics.AcousticProblem(grid=grid, eta=be.array(eta), wave=be.array(_rand(N, 6)), k0=6.0,
+                                        volume=1.0)
+    op = em.acoustics.AcousticOperator(prob)
+    u = be.array(_rand((2,) + N, 7))
+    assert np.array_equal(_host(op.matvec_window(u, lo, n)), _host(op.matvec(u)))
+
+
 def test_restricted_solves_match_the_oracle_full_solve(em):
     """SupportRestricted(BiCGStab) and SupportRestricted(MixedBiCGStab) reach
     the requested true relative residual of the FULL FP64 operator and agree
