@@ -1,14 +1,22 @@
 """Material sampling on the grid (src/em3d/geometry.py:11-252 of the reference).
 
-Masks and weights are evaluated on the host in float64 (as the reference does
-for every backend) and the resulting contrast is copied to the device once.
+Each shape answers three questions about a grid given by its cell-centre axes:
+which cell centres lie inside it (``cell_inside``), what fraction of each cell
+it covers (``cell_fraction``) and its exact volume (``volume``).  Both shapes
+are products or sums of one-axis terms, so they are evaluated on the three
+axes and combined by broadcasting, never on a meshgrid.  The tests keep the
+reference's semantics: centres are compared in float64 with the tie tolerance
+64 eps max(1, L) (geometry.py:62-64), boundaries inclusive, and an ellipsoid
+sums its three squared normalised offsets in x, y, z order, so masks equal
+the reference's bit for bit.  The resulting contrast is built on the host in
+complex128 and copied to the device once.
 """
 from __future__ import annotations
 
 from dataclasses import dataclass
 from enum import Enum
 from math import pi
-from typing import Union
+from typing import Sequence, Union
 
 import numpy as np
 
@@ -20,9 +28,25 @@ class SamplingMode(str, Enum):
     VOLUME_FRACTION = "volume_fraction"
 
 
+def _outer(parts, op):
+    """Combine three per-axis vectors into an (Nx, Ny, Nz) array with ``op``,
+    x first: op(op(a, b), c)."""
+    a, b, c = parts
+    return op(op(a[:, None, None], b[None, :, None]), c[None, None, :])
+
+
 @dataclass(frozen=True)
 class FullDomain:
-    pass
+    """The whole computational box."""
+
+    def cell_inside(self, axes: Sequence[np.ndarray], tol: float) -> np.ndarray:
+        return np.ones(tuple(len(a) for a in axes), dtype=bool)
+
+    def cell_fraction(self, axes: Sequence[np.ndarray], steps) -> np.ndarray:
+        return np.ones(tuple(len(a) for a in axes), dtype=np.float64)
+
+    def volume(self, L) -> float:
+        return float(np.prod(L))
 
 
 @dataclass(frozen=True)
@@ -34,6 +58,24 @@ class AxisAlignedBox:
         if any(float(v) <= 0.0 for v in self.size):
             raise ValueError(f"box sizes must be positive, got {self.size!r}")
 
+    def cell_inside(self, axes, tol):
+        # |x - c| <= s/2 + tol on every axis: a product of three 1-D tests
+        half = [float(s) / 2.0 + tol for s in self.size]
+        return _outer([np.abs(axes[a] - float(self.center[a])) <= half[a] for a in range(3)], np.logical_and)
+
+    def cell_fraction(self, axes, steps):
+        # the overlap of a cell with the box is the product of its three 1-D overlaps
+        parts = []
+        for a in range(3):
+            h, c, s = float(steps[a]), float(self.center[a]), float(self.size[a])
+            lo = np.maximum(axes[a] - h / 2.0, c - s / 2.0)
+            hi = np.minimum(axes[a] + h / 2.0, c + s / 2.0)
+            parts.append(np.clip(np.maximum(0.0, hi - lo) / h, 0.0, 1.0))
+        return _outer(parts, np.multiply)
+
+    def volume(self, L) -> float:
+        return float(np.prod(self.size))
+
 
 @dataclass(frozen=True)
 class Ellipsoid:
@@ -44,72 +86,53 @@ class Ellipsoid:
         if any(float(v) <= 0.0 for v in self.radii):
             raise ValueError(f"ellipsoid radii must be positive, got {self.radii!r}")
 
+    def cell_inside(self, axes, tol):
+        # sum of the squared normalised offsets, x + y first, then + z
+        q = [((axes[a] - float(self.center[a])) / float(self.radii[a])) ** 2 for a in range(3)]
+        return _outer(q, np.add) <= 1.0 + tol
+
+    def cell_fraction(self, axes, steps):
+        raise NotImplementedError("Ellipsoid has no volume-fraction rule; sample it at cell centres "
+                                  "(SamplingMode.CELL_CENTER)")
+
+    def volume(self, L) -> float:
+        return float(4.0 * pi * np.prod(self.radii) / 3.0)
+
 
 Geometry = Union[FullDomain, AxisAlignedBox, Ellipsoid]
 
 
+def _shape(geometry):
+    if not isinstance(geometry, (FullDomain, AxisAlignedBox, Ellipsoid)):
+        raise TypeError(f"unsupported geometry {type(geometry)!r}")
+    return geometry
+
+
 def _host_axes(grid: Grid):
+    """Cell-centre axes in float64 (the grid keeps them in its real dtype)."""
     return [np.asarray(a, dtype=np.float64) for a in grid._host]
 
 
 def _tolerance(grid: Grid) -> float:
-    scale = max(1.0, *(abs(float(v)) for v in grid.L))
-    return 64.0 * np.finfo(np.float64).eps * scale
+    return 64.0 * np.finfo(np.float64).eps * max(1.0, *(abs(float(v)) for v in grid.L))
 
 
 def center_mask(geometry: Geometry, grid: Grid) -> np.ndarray:
-    """Host mask of cell centres inside ``geometry`` (geometry.py:67-93)."""
-    if isinstance(geometry, FullDomain):
-        return np.ones(grid.N, dtype=bool)
-    X, Y, Z = np.meshgrid(*_host_axes(grid), indexing="ij")
-    tol = _tolerance(grid)
-    if isinstance(geometry, AxisAlignedBox):
-        out = np.ones(grid.N, dtype=bool)
-        for c, c0, s in zip((X, Y, Z), geometry.center, geometry.size):
-            out &= np.abs(c - float(c0)) <= float(s) / 2.0 + tol
-        return out
-    if isinstance(geometry, Ellipsoid):
-        v = np.zeros(grid.N, dtype=np.float64)
-        for c, c0, r in zip((X, Y, Z), geometry.center, geometry.radii):
-            v += ((c - float(c0)) / float(r)) ** 2
-        return v <= 1.0 + tol
-    raise TypeError(f"unsupported geometry {type(geometry)!r}")
-
-
-def _overlap(centres, step, fc, fs):
-    left, right = centres - step / 2.0, centres + step / 2.0
-    ov = np.maximum(0.0, np.minimum(right, fc + fs / 2.0) - np.maximum(left, fc - fs / 2.0))
-    return np.clip(ov / step, 0.0, 1.0)
+    """Host mask of the cell centres inside ``geometry`` (geometry.py:67-93)."""
+    return _shape(geometry).cell_inside(_host_axes(grid), _tolerance(grid))
 
 
 def volume_fractions(geometry: Geometry, grid: Grid) -> np.ndarray:
-    """Cell occupation fractions (geometry.py:114-148)."""
-    if isinstance(geometry, FullDomain):
-        return np.ones(grid.N, dtype=np.float64)
-    if isinstance(geometry, AxisAlignedBox):
-        axes = _host_axes(grid)
-        f = [_overlap(axes[a], float(grid.L[a]) / grid.N[a], float(geometry.center[a]), float(geometry.size[a]))
-             for a in range(3)]
-        return f[0][:, None, None] * f[1][None, :, None] * f[2][None, None, :]
-    if isinstance(geometry, Ellipsoid):
-        raise NotImplementedError("volume-fraction sampling for Ellipsoid is not implemented; "
-                                  "use SamplingMode.CELL_CENTER or add an explicit quadrature rule")
-    raise TypeError(f"unsupported geometry {type(geometry)!r}")
+    """Fraction of every cell that ``geometry`` covers (geometry.py:114-148)."""
+    return _shape(geometry).cell_fraction(_host_axes(grid), grid.h)
 
 
 def exact_volume(geometry: Geometry, grid: Grid) -> float:
-    if isinstance(geometry, FullDomain):
-        return float(np.prod(grid.L))
-    if isinstance(geometry, AxisAlignedBox):
-        return float(np.prod(geometry.size))
-    if isinstance(geometry, Ellipsoid):
-        return float(4.0 * pi * np.prod(geometry.radii) / 3.0)
-    raise TypeError(f"unsupported geometry {type(geometry)!r}")
+    return _shape(geometry).volume(grid.L)
 
 
 def sampling_weights(geometry: Geometry, grid: Grid, mode: SamplingMode = SamplingMode.CELL_CENTER) -> np.ndarray:
-    mode = SamplingMode(mode)
-    if mode is SamplingMode.CELL_CENTER:
+    if SamplingMode(mode) is SamplingMode.CELL_CENTER:
         return center_mask(geometry, grid).astype(np.float64)
     return volume_fractions(geometry, grid)
 
