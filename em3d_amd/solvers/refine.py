@@ -71,7 +71,9 @@ class MixedBiCGStab:
         self.preconditioner = preconditioner   # FNO: inner solves become right-preconditioned
         self.preconditioned_max_iter = int(preconditioned_max_iter)   # then plain inner solves take over
 
-    def solve(self, operator, rhs) -> SolverResult:
+    def solve(self, operator, rhs, *, rhs_norm: float | None = None) -> SolverResult:
+        """``rhs_norm``: |rhs| when the caller already holds it (one device
+        read-back fewer); it must be the FP64 norm of ``rhs``."""
         cfg = self.cfg
         be = operator.backend
         if be.precision is not Precision.DOUBLE:
@@ -84,7 +86,7 @@ class MixedBiCGStab:
         if self.preconditioner is not None:
             from .flexible import attach
             pc = attach(self.preconditioner, comp)
-        fn = norm(f)
+        fn = norm(f) if rhs_norm is None else float(rhs_norm)
         x = torch.zeros_like(f)
         if fn == 0.0:
             return SolverResult(u=x, iterations=0, residual_history=[0.0], converged=True,

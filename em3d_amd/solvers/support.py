@@ -23,6 +23,7 @@ from dataclasses import replace
 from ..support import support_operator
 from ._vec import dots
 from .base import SolverResult
+from .refine import MixedBiCGStab
 
 
 class SupportRestricted:
@@ -59,7 +60,9 @@ class SupportRestricted:
                                 residual_action_counts=[0], status="converged", true_final_residual=0.0,
                                 extras={"support": info})
         scale = fbn / fn
-        res = self._inner(float(self.solver.cfg.rtol) / scale).solve(sub, fb)
+        inner = self._inner(float(self.solver.cfg.rtol) / scale)
+        # the mixed solver takes |f_box| as read back above instead of reading it again
+        res = inner.solve(sub, fb, rhs_norm=fbn) if isinstance(inner, MixedBiCGStab) else inner.solve(sub, fb)
         # A u~ (u~ = u_box extended by zero), its forward transforms on the box alone
         y = operator.matvec_window(sub.extend(res.u), sub.box.lo, sub.box.n)
         u = sub.extend(res.u, f, y, out=y)                   # u_box inside, f - A u~ outside
