@@ -350,11 +350,16 @@ struct FastS {
     const bool ok = w < a.W;
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)q * a.in_qs;
     const unsigned lane = (unsigned)(((long long)j * a.W + w) * sizeof(cpx<T>));
+    // the row-window test only in the (uniform) branch of a windowed forward pass
+    auto load = [&](auto win) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m)
-      s.v[m] = (m < MA && ok && s_live(a, j + m * R2))
-                   ? ld_nt<NTS && (INV ? E3_SI_LD_NT : E3_SF_LD_NT)>(at_offset(in + (size_t)m * R2 * a.W, lane))
-                   : czero<T>();
+      for (int m = 0; m < R1; ++m)
+        s.v[m] = (m < MA && ok && (!decltype(win)::value || s_live(a, j + m * R2)))
+                     ? ld_nt<NTS && (INV ? E3_SI_LD_NT : E3_SF_LD_NT)>(at_offset(in + (size_t)m * R2 * a.W, lane))
+                     : czero<T>();
+    };
+    if (!INV && a.wr1) load(std::true_type{});
+    else load(std::false_type{});
     fft_reg<T, R1, INV>(s.v);
 #pragma unroll
     for (int k = 0; k < R1; ++k) lds[(k * R2 + j) * TCP + col] = s.v[k];
@@ -576,11 +581,15 @@ struct FastX {
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
       const T sg = comp_sign(c, q);
       cpx<T> v[RA];
+      auto load = [&](auto win) {
 #pragma unroll
-      for (int m = 0; m < RA; ++m)
-        v[m] = (m < RA / 2 && lv && x_live(a, jj + m * (L / RA)))
-                   ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) * sg
-                   : czero<T>();
+        for (int m = 0; m < RA; ++m)
+          v[m] = (m < RA / 2 && lv && (!decltype(win)::value || x_live(a, jj + m * (L / RA))))
+                     ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b)) * sg
+                     : czero<T>();
+      };
+      if (a.wx1) load(std::true_type{});   // uniform: a windowed forward transform
+      else load(std::false_type{});
       fft_reg<T, RA, false>(v);
 #pragma unroll
       for (int m = 0; m < RA; ++m) cell(t, jj * RA + m, q.col) = v[m];
@@ -597,11 +606,15 @@ struct FastX {
     for (int bb = 0; bb < BA; ++bb) {
       const int jj = q.j + bb * NTC;
       const unsigned lane_b = (unsigned)(((long long)jj * a.W + q.w) * sizeof(cpx<T>));
+      auto load = [&](auto win) {
 #pragma unroll
-      for (int m = 0; m < RA / 2; ++m)
-        s.pre[bb][m] = (lv && x_live(a, jj + m * (L / RA)))
-                           ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b))
-                           : czero<T>();
+        for (int m = 0; m < RA / 2; ++m)
+          s.pre[bb][m] = (lv && (!decltype(win)::value || x_live(a, jj + m * (L / RA))))
+                             ? ld_nt<NTX && E3_X_LD_NT>(at_offset(d + (size_t)m * (L / RA) * a.W, lane_b))
+                             : czero<T>();
+      };
+      if (a.wx1) load(std::true_type{});   // uniform: a windowed forward transform
+      else load(std::false_type{});
     }
   }
   E3_HD void fwd_a_pre(int c, int tid, int h, St& s, int buf) const {
@@ -1017,9 +1030,15 @@ struct FastZF {
     const bool ok = p0 + pl < a.P;
     const long long p = z_pencil(a, p0 + pl);
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)b * a.in_bs;
+    auto load = [&](auto win) {
 #pragma unroll
-    for (int m = 0; m < R1; ++m)
-      v[m] = (m < MA && ok && z_live(a, j + m * R2)) ? contrast_row<T, NTZ, NTU>(a, in, c, p, j + m * R2) : czero<T>();
+      for (int m = 0; m < R1; ++m)
+        v[m] = (m < MA && ok && (!decltype(win)::value || z_live(a, j + m * R2)))
+                   ? contrast_row<T, NTZ, NTU>(a, in, c, p, j + m * R2)
+                   : czero<T>();
+    };
+    if (a.wz1) load(std::true_type{});   // uniform: a windowed forward transform
+    else load(std::false_type{});
   }
   // stage A input with a fused Krylov prologue (KRY_P / KRY_S, iso or diag
   // contrast): the update of each live element, written back, then chi.  The
