@@ -350,16 +350,16 @@ struct FastS {
     const bool ok = w < a.W;
     const cpx<T>* in = (const cpx<T>*)a.in + (size_t)q * a.in_qs;
     const unsigned lane = (unsigned)(((long long)j * a.W + w) * sizeof(cpx<T>));
-    // the row-window test only in the (uniform) branch of a windowed forward pass
-    auto load = [&](auto win) {
+    // input rows outside a forward pass's window [r0, r1) read as zero (one
+    // loop: two code paths here leave the mixed-radix row values in scratch)
+    constexpr bool NTL = NTS && (INV ? E3_SI_LD_NT : E3_SF_LD_NT);
+    const int r0 = (!INV && a.wr1) ? a.wr0 : 0, r1 = (!INV && a.wr1) ? a.wr1 : L;
 #pragma unroll
-      for (int m = 0; m < R1; ++m)
-        s.v[m] = (m < MA && ok && (!decltype(win)::value || s_live(a, j + m * R2)))
-                     ? ld_nt<NTS && (INV ? E3_SI_LD_NT : E3_SF_LD_NT)>(at_offset(in + (size_t)m * R2 * a.W, lane))
-                     : czero<T>();
-    };
-    if (!INV && a.wr1) load(std::true_type{});
-    else load(std::false_type{});
+    for (int m = 0; m < R1; ++m) {
+      const int row = j + m * R2;
+      const bool live = INV ? (m < MA && ok) : (m < MA && ok && row >= r0 && row < r1);
+      s.v[m] = live ? ld_nt<NTL>(at_offset(in + (size_t)m * R2 * a.W, lane)) : czero<T>();
+    }
     fft_reg<T, R1, INV>(s.v);
 #pragma unroll
     for (int k = 0; k < R1; ++k) lds[(k * R2 + j) * TCP + col] = s.v[k];
