@@ -327,6 +327,32 @@ def test_restricted_solves_match_the_oracle_full_solve(em):
         assert res.residual_history[-1] < rtol
 
 
+@pytest.mark.parametrize("name", ["SIM", "TwoStep"])
+def test_restricted_host_solvers(em, name):
+    """SupportRestricted around the host-driven solvers (SIM with the
+    operator's (mu, R) enclosure, TwoStep with its rmatvec on the box): the
+    full operator's true residual meets the tolerance and the field agrees
+    with the full-grid BiCGStab solution."""
+    N, k = (24, 20, 16), 1.0
+    _say(f"[support {name}] N={N}")
+    be = _be(em, "double")
+    grid = em.Grid(N=N, L=(1.0, 1.0, 1.0), center=(0.0, 0.0, 0.0), backend=be)
+    chi = em.geometry.sample_contrast_tensor(grid, background_eps_r=1.0, feature_eps_r=1.3 + 0.1j,
+                                             geometry=em.geometry.Ellipsoid(center=(0.05, 0.0, -0.1),
+                                                                            radii=(0.2, 0.15, 0.25)))
+    wave = em.flat_wave_vec(grid, k=k, orient=(0, 0, 1), amplitude=(1, 0, 0))
+    prob = em.Problem(grid=grid, eps_tensor=chi, wave=wave, k0=k, volume=1.0)
+    op = em.Operator(prob)
+    cfg = em.SolverConfig(max_iter=400, rtol=1e-7, mu=1.0 + 0j, radius=0.5)
+    j = em.judge_solve(em.SupportRestricted(getattr(em, name)(cfg)), op, wave)
+    res = j.result
+    _say(f"  {res.status} it={res.iterations} true={j.true_residual:.3e} box={res.extras['support']}")
+    assert res.extras["support"]["fraction"] < 0.5
+    assert res.status == "converged" and j.qualified and j.true_residual < 1e-7
+    ref = em.BiCGStab(em.SolverConfig(max_iter=400, rtol=1e-11)).solve(op, wave)
+    assert _rel(_host(res.u), _host(ref.u)) < 1e-5
+
+
 def test_restricted_acoustic_solve(em):
     from oracle import solvers as osolv
     from oracle import vie
