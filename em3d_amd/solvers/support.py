@@ -20,14 +20,14 @@ from __future__ import annotations
 import copy
 from dataclasses import replace
 
-from ..support import support_operator
+from ..support import DEFAULT_MAX_FRACTION, support_operator
 from ._vec import dots
 from .base import SolverResult
 from .refine import MixedBiCGStab
 
 
 class SupportRestricted:
-    def __init__(self, solver, *, max_fraction: float = 0.5):
+    def __init__(self, solver, *, max_fraction: float = DEFAULT_MAX_FRACTION):
         if not (0.0 < max_fraction <= 1.0):
             raise ValueError(f"max_fraction must lie in (0, 1], got {max_fraction}")
         self.solver = solver
