@@ -1423,6 +1423,15 @@ bool fast_z_len(Run& run, const ZArgs& a) {
   }
   if (a.variant == small_variant(2, L, sizeof(T) == 8 ? 1 : 0))
     return fast_z_cfg<T, L, ZSmall<ZVar<T, L, 0>, L>, true, NT>(run, a);
+  // From 256 points on, the plain forward pass of the default tiles runs the
+  // small tiles too: its default block holds so large an LDS tile that only
+  // two blocks fit a CU, and smaller blocks fit more wavefronts (measured,
+  // same box: P1 of the acoustic 256^3 matvec 200 -> 168 us, of the FP64 EM
+  // 128^3 one 92 -> 88 us).  The inverse pass keeps its default tiles (the
+  // small ones are slower there: FP64 75 -> 79 us, acoustic 111 -> 127 us),
+  // and so does the fused Krylov forward pass (the FP64 solve lost 0.5 %).
+  if (a.variant == 0 && a.inv == 0 && a.k.op == KRY_NONE && L >= 256)
+    return fast_z_cfg<T, L, ZSmall<ZVar<T, L, 0>, L>, true, NT>(run, a);
   return fast_z_var<T, L, 0, NT>(run, a);
 }
 
