@@ -212,16 +212,17 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "em_matvec_128_c64_hbm_traffic.
 TRACE_SUMMARY = os.path.join(ROOT, "profiles", "em_matvec_128_c64_kernel_trace.txt")
 
 
-def measured_traffic(kernel, N, n_chi, es):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), for the workload it was
-    measured on (128^3, diagonal chi, complex64); None otherwise."""
+def measured_traffic(kernel, N, n_chi, es, variant=0):
+    """HBM bytes per launch of `kernel` (kernel variant `variant`) from the
+    committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE passes,
+    copy-calibrated), for the workload it was measured on (128^3, diagonal
+    chi, complex64); None otherwise."""
     if (N, n_chi, es) != (128, 3, 8) or not os.path.exists(TRAFFIC_SUMMARY):
         return None
     for line in open(TRAFFIC_SUMMARY):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
-            return int(f[2])
+        if len(f) == 5 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and f[3] == str(variant):
+            return int(f[4])
     return None
 
 
@@ -829,9 +830,10 @@ def main():
                                                 "tuned_ms": round(ms_per, 5)},
         "roofline": {"bound": "hbm", "kernel": PASS_NAMES[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     # the committed PMC summary was measured with the default x-pass variant
-                     "traffic": measured_traffic(PASS_NAMES[dom], N, n_chi, es)
-                     if tuned is None or tuned["variants"][1] == 0 else None,
+                     # the committed PMC summary holds every x-pass variant the autotuner may pick
+                     "traffic": measured_traffic(PASS_NAMES[dom], N, n_chi, es,
+                                                 0 if tuned is None or PASS_NAMES[dom] != "x_fwd_khat_inv"
+                                                 else tuned["variants"][1]),
                      "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT),
                      "time_source": "hipEvents recorded before and after each launch on the launch stream",
                      "trace_summary": os.path.relpath(TRACE_SUMMARY, ROOT),
