@@ -452,8 +452,16 @@ struct XR {
 #ifndef E3_X_PREFETCH
 #define E3_X_PREFETCH 1
 #endif
+// spectral products scheduled together (their octant reads in flight at
+// once): two for complex64; one for complex128, whose x pass runs two
+// wavefronts per SIMD (measured at 128^3, same box: FP64 x pass 288 -> 277 us
+// with one, 295 us with four; complex64: the default plan unchanged with one
+// or four, the 16 x 1 x 16 plan 7 % slower with one)
 #ifndef E3_X_MULGROUP
 #define E3_X_MULGROUP 2
+#endif
+#ifndef E3_X_MULGROUP_F64
+#define E3_X_MULGROUP_F64 1
 #endif
 template <typename T, int L, int C, int TC_ = 128 / (int)sizeof(cpx<T>), bool STAGE_ = true, int WAVES_ = 2,
           class RX = XRadix<L>, bool PAIR_ = false, bool NTX = false, bool RSPLIT_ = false>
@@ -490,7 +498,7 @@ struct FastX {
   static constexpr bool STAGE = STAGE_ && (size_t)(NSEG * SEG + L + KS) * sizeof(cpx<T>) <= 160 * 1024;
   static constexpr int MIN_WAVES_PER_EU = WAVES_;
   static constexpr int LDS = NSEG * SEG + L + (STAGE ? KS : 0);
-  static constexpr int kMulGroup = E3_X_MULGROUP;      // spectral products scheduled together
+  static constexpr int kMulGroup = sizeof(T) == 8 ? E3_X_MULGROUP_F64 : E3_X_MULGROUP;
   static_assert(RA * (RB > 1 ? RB : 1) * RC == L, "radix plan");
   static_assert(RC % RA == 0 && (RB == 1 || RC % RB == 0), "stage radices must divide the last one");
   struct St {
