@@ -182,7 +182,7 @@ def test_generic_and_fast_paths_agree(lib):
 VARIANT_CASES = [((128, 2, 8), 1, "single"), ((96, 2, 8), 1, "single"), ((64, 2, 8), 1, "single"),  # x: 256/192/128
                  ((2, 128, 8), 0, "single"), ((2, 96, 8), 0, "single"),          # strided passes at 256, 192
                  ((2, 2, 128), 2, "single"), ((2, 2, 96), 2, "single"),          # z passes at 256, 192
-                 ((128, 2, 8), 1, "double")]                                     # FP64 x pass at 256
+                 ((128, 2, 8), 1, "double"), ((2, 128, 8), 0, "double")]         # FP64 x and strided at 256
 
 
 @pytest.mark.parametrize("N,kind,precision", VARIANT_CASES)

@@ -182,11 +182,11 @@ def test_packed_members_vs_oracle(em):
 
 
 # (N, pass kind, precision): x pass at 256 / 192 / 128, strided passes at
-# 256 / 192, z passes at 256 / 192, the FP64 x pass at 256 -- every variant
+# 256 / 192, z passes at 256 / 192, the FP64 x and strided passes at 256 -- every variant
 # the autotuner may pick, and the generic kernels (-1)
 VARIANT_CASES = [((128, 2, 8), 1, "single"), ((96, 2, 8), 1, "single"), ((64, 2, 8), 1, "single"),
                  ((2, 128, 8), 0, "single"), ((2, 96, 8), 0, "single"), ((2, 2, 128), 2, "single"),
-                 ((2, 2, 96), 2, "single"), ((128, 2, 8), 1, "double")]
+                 ((2, 2, 96), 2, "single"), ((128, 2, 8), 1, "double"), ((2, 128, 8), 0, "double")]
 
 
 @pytest.mark.parametrize("N,kind,precision", VARIANT_CASES)
